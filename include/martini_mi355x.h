@@ -1,0 +1,153 @@
+/*
+ * martini_mi355x.h -- C ABI of the MI355X source-insertion library.
+ *
+ * Replaces the hot loop of Martini.insert_source_in_cube
+ * (martini/martini.py:285-407, _evaluate_pixel_spectrum :243-283) together
+ * with the per-particle kernel integrals (martini/sph_kernels.py:85-119 and
+ * the concrete _kernel_integral methods) and the spectra
+ * (martini/spectral_models.py:63-200, :347-425, :510-570).
+ *
+ * Conventions
+ *   - All pointers named d_* are device pointers (HBM); all calls are
+ *     asynchronous on the given stream unless documented otherwise.
+ *   - Return value: 0 on success, a negative MI_ERR_* code on failure.
+ *     mi_strerror() turns it into text.  No exception crosses the ABI.
+ *   - Nothing is allocated inside the calls: the caller passes the
+ *     workspace, sized by mi_insert_workspace_size().
+ *   - Calls are reentrant for distinct (workspace, stream) pairs.
+ *   - The cube is FP32, row-major [nx][ny][nc] over the padded extent, and
+ *     is ACCUMULATED (+=), as the reference accumulates into DataCube._array
+ *     (martini/martini.py:338-362).
+ */
+#ifndef MARTINI_MI355X_H
+#define MARTINI_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_ABI_VERSION 1
+
+/* Error codes. */
+#define MI_OK 0
+#define MI_ERR_ARGUMENT (-1)   /* null pointer, negative size, bad descriptor */
+#define MI_ERR_WORKSPACE (-2)  /* workspace too small for this input */
+#define MI_ERR_DEVICE (-3)     /* a HIP runtime call failed */
+#define MI_ERR_UNSUPPORTED (-4) /* kernel/spectrum kind not handled */
+#define MI_ERR_LIMIT (-5)      /* input exceeds a documented size limit */
+
+/* Kernel kinds (low 4 bits of mi_particle.r_kind).  Concrete kernels of
+ * martini/sph_kernels.py: _CubicSplineKernel :724, _WendlandC2Kernel :340,
+ * _WendlandC6Kernel :481, _QuarticSplineKernel :1402, _GaussianKernel :897,
+ * DiracDeltaKernel :1083.  Adaptive kernels are resolved per particle on the
+ * host (_AdaptiveKernel._init_sm_lengths :1241-1274). */
+#define MI_KERNEL_CUBIC 0
+#define MI_KERNEL_WC2 1
+#define MI_KERNEL_WC6 2
+#define MI_KERNEL_QUARTIC 3
+#define MI_KERNEL_GAUSS 4
+#define MI_KERNEL_DIRAC 5
+
+/* Spectral models (martini/spectral_models.py): GaussianSpectrum :304,
+ * DiracDeltaSpectrum :488. */
+#define MI_SPECTRUM_GAUSS 0
+#define MI_SPECTRUM_DIRAC 1
+
+/* Insertion methods. */
+#define MI_METHOD_PLAIN_F32 0   /* FP32 VALU tile gather */
+#define MI_METHOD_MFMA_BF16X3 1 /* BF16 split-precision MFMA contraction */
+
+/*
+ * One particle after the host's FP64 setup (projection, smoothing length,
+ * adaptive kernel choice, pruning).  48 bytes, 16-byte aligned.
+ *
+ * Pixel position x = ix + fx, y = iy + fy (0-based, padded cube pixels;
+ * SPHSource.pixcoords, martini/sources/sph_source.py:328-362).  Splitting off
+ * the integer part keeps the FP32 offsets x - i exact to 2^-24 px.
+ * Mask: |i - x| <= r and |j - y| <= r (martini/martini.py:271-274).
+ * Dirac-kernel particles carry their single pixel as (ix, iy), fx = fy = 0,
+ * r = 0, or r < 0 when no pixel receives weight.
+ *
+ * Spectrum: channel edges e_k = e0 + k*de (uniform in velocity; both VRAD and
+ * radio-FREQ cubes are).  u = (v - e0)/de = k0 + fu.  Channel c receives
+ * amp * 0.5*(erf(s*(u-c)) - erf(s*(u-c-1))) for a Gaussian line with
+ * s = |de|/(sqrt(2) sigma), or amp for c in [c_lo, c_hi] for a Dirac line.
+ * [c_lo, c_hi] is the window outside which the line is below 1e-10 of its
+ * flux (Gaussian) or zero (Dirac).
+ */
+typedef struct mi_particle {
+    int32_t ix, iy;
+    float fx, fy;
+    float h;         /* effective kernel size h_px * rescale, pixels */
+    int32_t r_kind;  /* (r << 8) | (gaussian table index << 4) | kernel kind */
+    int32_t c_lo, c_hi;
+    int32_t k0;
+    float fu;
+    float s;
+    float amp;       /* mHI / d^2 / |dv| / 2.36e5 * output scale (Jy per unit weight) */
+} mi_particle;
+
+typedef struct mi_cube_desc {
+    int32_t nx, ny, nc;      /* padded cube extent */
+    int32_t spectrum;        /* MI_SPECTRUM_* */
+    float gauss_t[4];        /* Gaussian-kernel truncations by table index */
+    float gauss_inv_norm[4]; /* 1 / _GaussianKernel.norm by table index */
+} mi_cube_desc;
+
+/* Library ABI version (MI_ABI_VERSION). */
+int mi_abi_version(void);
+
+/* Text for an MI_ERR_* code. */
+const char* mi_strerror(int code);
+
+/*
+ * Number of (particle, cell) pairs the binning of these particles produces:
+ * a cell is a 16 x 16 pixel tile times a 64-channel block.  Host-side count
+ * from host records (no device access); the value sizes the workspace.
+ */
+int64_t mi_count_pairs_host(const mi_particle* h_particles, int64_t n,
+                            const mi_cube_desc* desc);
+
+/* Workspace bytes for mi_insert with n particles producing n_pairs pairs. */
+int mi_insert_workspace_size(const mi_cube_desc* desc, int64_t n, int64_t n_pairs,
+                             size_t* bytes);
+
+/*
+ * Bin, sort and insert: cube += sum over particles of weight * spectrum.
+ * Replaces _BaseMartini._insert_source_in_cube's pixel loop
+ * (martini/martini.py:327-362).  d_cube must hold nx*ny*nc floats.
+ * n_pairs must equal mi_count_pairs_host() for these particles (or bound it:
+ * the call fails with MI_ERR_WORKSPACE, leaving the cube untouched, if the
+ * device count exceeds it).  Synchronises the stream once, after binning.
+ * method: MI_METHOD_*.
+ */
+int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+              const mi_cube_desc* desc, float* d_cube, void* d_workspace,
+              size_t workspace_bytes, int method, void* stream);
+
+/*
+ * Kernel integrals at n offsets: out[k] = W(dx[k], dy[k], h[k]) for one
+ * concrete kernel kind (gaussian: truncation t, 1/norm).  Device arrays.
+ * Replaces _BaseSPHKernel._px_weight / _kernel_integral
+ * (martini/sph_kernels.py:85-119).
+ */
+int mi_kernel_weights(int kind, float gauss_t, float gauss_inv_norm, const float* d_dx,
+                      const float* d_dy, const float* d_h, float* d_out, int64_t n,
+                      void* stream);
+
+/*
+ * Spectra: out[p*nc + c] = amp_p * profile_p(c) for n particles (N x Nc,
+ * FP32), from the same fields insertion uses.  Replaces
+ * _BaseSpectrum.init_spectra's evaluation (martini/spectral_models.py:63-147).
+ */
+int mi_spectra(const mi_particle* d_particles, int64_t n, const mi_cube_desc* desc,
+               float* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARTINI_MI355X_H */

@@ -1,0 +1,164 @@
+"""ctypes binding of libmartini_mi355x.so (the C ABI in include/martini_mi355x.h).
+
+The library is required: importing the compute entry points without it, or
+calling them without a GPU, raises.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmartini_mi355x.so")
+
+MI_OK = 0
+MI_ERR_WORKSPACE = -2
+
+KERNEL_CUBIC, KERNEL_WC2, KERNEL_WC6, KERNEL_QUARTIC, KERNEL_GAUSS, KERNEL_DIRAC = range(6)
+SPECTRUM_GAUSS, SPECTRUM_DIRAC = 0, 1
+METHOD_PLAIN_F32, METHOD_MFMA_BF16X3 = 0, 1
+METHODS = {"plain": METHOD_PLAIN_F32, "fp32": METHOD_PLAIN_F32,
+           "mfma": METHOD_MFMA_BF16X3, "bf16x3": METHOD_MFMA_BF16X3}
+
+# struct mi_particle (48 bytes)
+PARTICLE_DTYPE = np.dtype([
+    ("ix", "<i4"), ("iy", "<i4"), ("fx", "<f4"), ("fy", "<f4"), ("h", "<f4"),
+    ("r_kind", "<i4"), ("c_lo", "<i4"), ("c_hi", "<i4"), ("k0", "<i4"), ("fu", "<f4"),
+    ("s", "<f4"), ("amp", "<f4"),
+])
+assert PARTICLE_DTYPE.itemsize == 48
+
+
+class CubeDesc(ctypes.Structure):
+    """struct mi_cube_desc."""
+
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nc", ctypes.c_int32),
+                ("spectrum", ctypes.c_int32), ("gauss_t", ctypes.c_float * 4),
+                ("gauss_inv_norm", ctypes.c_float * 4)]
+
+
+class MartiniLibError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m martini_amd.build_native` "
+            "(the insertion path has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, f32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
+    L.mi_abi_version.restype = ctypes.c_int
+    L.mi_strerror.restype = ctypes.c_char_p
+    L.mi_strerror.argtypes = [ctypes.c_int]
+    L.mi_count_pairs_host.restype = i64
+    L.mi_count_pairs_host.argtypes = [vp, i64, ctypes.POINTER(CubeDesc)]
+    L.mi_insert_workspace_size.restype = ctypes.c_int
+    L.mi_insert_workspace_size.argtypes = [ctypes.POINTER(CubeDesc), i64, i64,
+                                           ctypes.POINTER(ctypes.c_size_t)]
+    L.mi_insert.restype = ctypes.c_int
+    L.mi_insert.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, vp, ctypes.c_size_t,
+                            ctypes.c_int, vp]
+    L.mi_kernel_weights.restype = ctypes.c_int
+    L.mi_kernel_weights.argtypes = [ctypes.c_int, f32, f32, vp, vp, vp, vp, i64, vp]
+    L.mi_spectra.restype = ctypes.c_int
+    L.mi_spectra.argtypes = [vp, i64, ctypes.POINTER(CubeDesc), vp, vp]
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != MI_OK:
+        msg = lib().mi_strerror(rc).decode()
+        raise MartiniLibError(f"{what} failed: {msg} (code {rc})")
+
+
+def _torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("martini_amd's insertion path runs on the GPU and no GPU is visible")
+    return torch
+
+
+def stream_ptr(torch_stream=None):
+    torch = _torch_cuda()
+    s = torch_stream if torch_stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def make_desc(nx, ny, nc, spectrum, gauss_table=()) -> CubeDesc:
+    d = CubeDesc()
+    d.nx, d.ny, d.nc, d.spectrum = int(nx), int(ny), int(nc), int(spectrum)
+    for k in range(4):
+        d.gauss_t[k] = 0.0
+        d.gauss_inv_norm[k] = 1.0
+    for k, (t, norm) in enumerate(gauss_table):
+        d.gauss_t[k] = float(t)
+        d.gauss_inv_norm[k] = float(1.0 / norm)
+    return d
+
+
+def count_pairs(records: np.ndarray, desc: CubeDesc) -> int:
+    rec = np.ascontiguousarray(records)
+    n = lib().mi_count_pairs_host(rec.ctypes.data_as(ctypes.c_void_p), rec.size, ctypes.byref(desc))
+    if n < 0:
+        check(int(n), "mi_count_pairs_host")
+    return int(n)
+
+
+def workspace_bytes(desc: CubeDesc, n: int, n_pairs: int) -> int:
+    out = ctypes.c_size_t(0)
+    check(lib().mi_insert_workspace_size(ctypes.byref(desc), n, n_pairs, ctypes.byref(out)),
+          "mi_insert_workspace_size")
+    return int(out.value)
+
+
+def insert(d_records, n: int, n_pairs: int, desc: CubeDesc, d_cube, d_ws, method="plain",
+           stream=None) -> None:
+    """cube += insertion of n device records (torch tensors on the GPU)."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    rc = lib().mi_insert(ctypes.c_void_p(d_records.data_ptr()), n, n_pairs, ctypes.byref(desc),
+                         ctypes.c_void_p(d_cube.data_ptr()), ctypes.c_void_p(d_ws.data_ptr()),
+                         d_ws.numel() * d_ws.element_size(), m, stream_ptr(stream))
+    check(rc, "mi_insert")
+
+
+def kernel_weights(kind: int, dx, dy, h, gauss_t=0.0, gauss_norm=1.0) -> np.ndarray:
+    """W(dx, dy, h) for one concrete kernel kind, evaluated on the GPU (FP32)."""
+    torch = _torch_cuda()
+    dx = np.ascontiguousarray(dx, np.float32).ravel()
+    n = dx.size
+    dy = np.ascontiguousarray(np.broadcast_to(np.asarray(dy, np.float32).ravel(), (n,)))
+    h = np.ascontiguousarray(np.broadcast_to(np.asarray(h, np.float32).ravel(), (n,)))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    tx, ty, th = (torch.from_numpy(a).to(dev) for a in (dx, dy, h))
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    rc = lib().mi_kernel_weights(int(kind), float(gauss_t), float(1.0 / gauss_norm),
+                                 ctypes.c_void_p(tx.data_ptr()), ctypes.c_void_p(ty.data_ptr()),
+                                 ctypes.c_void_p(th.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                 n, stream_ptr())
+    check(rc, "mi_kernel_weights")
+    return out.cpu().numpy()
+
+
+def spectra(records: np.ndarray, desc: CubeDesc) -> np.ndarray:
+    """N x Nc spectra (Jy, FP32) of host records, evaluated on the GPU."""
+    torch = _torch_cuda()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    rec = torch.from_numpy(np.ascontiguousarray(records).view(np.uint8)).to(dev)
+    n = records.size
+    out = torch.empty((n, desc.nc), dtype=torch.float32, device=dev)
+    check(lib().mi_spectra(ctypes.c_void_p(rec.data_ptr()), n, ctypes.byref(desc),
+                           ctypes.c_void_p(out.data_ptr()), stream_ptr()), "mi_spectra")
+    return out.cpu().numpy()

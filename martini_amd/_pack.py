@@ -1,0 +1,168 @@
+"""Host FP64 setup -> ``mi_particle`` records (the host half of the boundary).
+
+Everything that needs FP64 happens here, once per particle:
+
+* pixel position split into an integer part and an FP32 fraction, so that
+  offsets x - i are exact to 2^-24 px on the device;
+* the mask half-width r = sm_ranges (martini/martini.py:271-274) and the
+  concrete kernel of each particle (adaptive choice, sph_kernels.py:1241-1274),
+  with h = h_px * rescale (sph_kernels.py:85-119);
+* Dirac-kernel particles resolved to their single pixel with the reference's
+  strict |dx| < 0.5 test (sph_kernels.py:1136-1165) in FP64;
+* the line-of-sight velocity in channel-edge units u = (v - e0)/de split as
+  k0 + fu, because channel widths sit near FP32 epsilon of the velocities
+  themselves;
+* the channel window outside which the Gaussian line is below erfc(4.5)/2
+  (2e-10) of its flux, or the Dirac line's channel(s) with the reference's
+  inclusive edge test (spectral_models.py:510-570);
+* amp = mHI / d^2 / |dv| / 2.36e5 (spectral_models.py:92-145) times the
+  output scale (1/px^2 converts Jy/pix^2 to Jy/arcsec^2, martini.py:364-366).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+
+WINDOW_ERF_ARG = 4.5
+MHI_TO_JY = 2.36e5
+_MAX_INDEX = 1 << 21  # |pixel coordinate| bound
+_MAX_RANGE = 1 << 22  # r is stored in 24 bits of r_kind; larger r already covers the cube
+
+
+def channel_grid(datacube):
+    """Velocity edges (m/s), first edge e0 and uniform step de (signed)."""
+    e = np.asarray(datacube.velocity_channel_edges, float)
+    nc = datacube.n_channels
+    return e, e[0], (e[-1] - e[0]) / nc
+
+
+def spectral_fields(vlos_kms, sigma_kms, mHI, dist_mpc, datacube, spectrum_kind,
+                    out_scale=1.0):
+    """k0, fu, s, amp, c_lo, c_hi per particle."""
+    e, e0, de = channel_grid(datacube)
+    nc = datacube.n_channels
+    v = np.asarray(vlos_kms, float) * 1e3
+    n = v.size
+    with np.errstate(invalid="ignore"):
+        u = (v - e0) / de
+    finite = np.isfinite(u)
+    uu = np.where(finite, u, 0.0)
+    k0 = np.floor(uu)
+    fu = uu - k0
+    amp = (np.asarray(mHI, float) * np.asarray(dist_mpc, float) ** -2.0
+           / (abs(de) / 1e3) / MHI_TO_JY * out_scale)
+    if spectrum_kind == _lib.SPECTRUM_GAUSS:
+        sig = np.broadcast_to(np.asarray(sigma_kms, float), (n,)) * 1e3
+        with np.errstate(divide="ignore"):
+            s = abs(de) / (np.sqrt(2.0) * sig)
+            half = WINDOW_ERF_ARG / s
+        c_lo = np.floor(uu - 1.0 - half) + 1.0
+        c_hi = np.ceil(uu + half) - 1.0
+    else:
+        s = np.zeros(n)
+        lo_e = np.minimum(e[:-1], e[1:])
+        hi_e = np.maximum(e[:-1], e[1:])
+        c_lo = np.full(n, np.inf)
+        c_hi = np.full(n, -np.inf)
+        for d in (-1.0, 0.0, 1.0):
+            c = np.clip(k0 + d, 0, nc - 1).astype(np.int64)
+            inside = (v >= lo_e[c]) & (v <= hi_e[c]) & finite & (k0 + d >= 0) & (k0 + d <= nc - 1)
+            c_lo = np.where(inside, np.minimum(c_lo, c), c_lo)
+            c_hi = np.where(inside, np.maximum(c_hi, c), c_hi)
+    c_lo = np.clip(np.where(finite, c_lo, nc), 0, nc)
+    c_hi = np.clip(np.where(finite, c_hi, -1), -1, nc - 1)
+    empty = c_lo > c_hi
+    c_lo = np.where(empty, 1, c_lo)
+    c_hi = np.where(empty, 0, c_hi)
+    return (k0.astype(np.int64), fu, s, amp, c_lo.astype(np.int64), c_hi.astype(np.int64))
+
+
+def kernel_fields(sph_kernel, pixcoords, nx, ny):
+    """ix, iy, fx, fy, h, r_kind per particle and the Gaussian (t, norm) table."""
+    kernels, which = sph_kernel._device_kernels()
+    kinds = np.array([K._mi_kind for K in kernels], np.int64)[which]
+    gauss = []
+    gidx_of_kernel = np.zeros(len(kernels), np.int64)
+    for ik, K in enumerate(kernels):
+        if K._mi_kind == _lib.KERNEL_GAUSS:
+            key = (float(K.truncate), float(K.norm))
+            if key not in gauss:
+                gauss.append(key)
+            gidx_of_kernel[ik] = gauss.index(key)
+    if len(gauss) > 4:
+        raise ValueError("at most 4 distinct Gaussian truncations per insertion")
+    gidx = gidx_of_kernel[which]
+    h_px = np.asarray(sph_kernel.sm_lengths, float)
+    h_eff = h_px * np.asarray(sph_kernel._rescale, float)
+    r = np.asarray(sph_kernel.sm_ranges, float)
+    x = np.asarray(pixcoords[0], float)
+    y = np.asarray(pixcoords[1], float)
+    with np.errstate(invalid="ignore"):
+        alive = (np.isfinite(x) & np.isfinite(y) & (r >= 0) & (x + r >= 0) & (y + r >= 0)
+                 & (x - r <= nx - 1) & (y - r <= ny - 1))
+    xs = np.where(alive, x, 0.0)
+    ys = np.where(alive, y, 0.0)
+    if np.any(np.abs(xs) >= _MAX_INDEX) or np.any(np.abs(ys) >= _MAX_INDEX):
+        raise ValueError("pixel coordinates beyond 2^21 cannot be represented")
+    ix = np.floor(xs)
+    iy = np.floor(ys)
+    fx = xs - ix
+    fy = ys - iy
+    rr = np.where(alive, np.minimum(r, float(_MAX_RANGE)), -1.0)
+    dirac = kinds == _lib.KERNEL_DIRAC
+    if dirac.any():
+        # the one pixel with |x - i| < 0.5 (strict) that the mask also admits
+        px_i = np.floor(xs + 0.5)
+        px_j = np.floor(ys + 0.5)
+        ok = ((np.abs(xs - px_i) < 0.5) & (np.abs(ys - px_j) < 0.5)
+              & (np.abs(xs - px_i) <= rr) & (np.abs(ys - px_j) <= rr)
+              & (px_i >= 0) & (px_i <= nx - 1) & (px_j >= 0) & (px_j <= ny - 1) & alive)
+        ix = np.where(dirac, px_i, ix)
+        iy = np.where(dirac, px_j, iy)
+        fx = np.where(dirac, 0.0, fx)
+        fy = np.where(dirac, 0.0, fy)
+        rr = np.where(dirac, np.where(ok, 0.0, -1.0), rr)
+    r_kind = (rr.astype(np.int64) << 8) | (gidx << 4) | kinds
+    return ix.astype(np.int64), iy.astype(np.int64), fx, fy, h_eff, r_kind, gauss
+
+
+def make_records(n, ix, iy, fx, fy, h, r_kind, k0, fu, s, amp, c_lo, c_hi) -> np.ndarray:
+    rec = np.empty(n, dtype=_lib.PARTICLE_DTYPE)
+    rec["ix"], rec["iy"] = ix, iy
+    rec["fx"], rec["fy"] = fx, fy
+    rec["h"] = h
+    rec["r_kind"] = r_kind
+    rec["c_lo"], rec["c_hi"] = c_lo, c_hi
+    rec["k0"] = np.clip(k0, -(1 << 30), 1 << 30)
+    rec["fu"] = fu
+    rec["s"] = s
+    rec["amp"] = amp
+    return rec
+
+
+def pack_particles(source, sph_kernel, spectral_model, datacube, out_scale=1.0):
+    """Records and cube descriptor for an initialised (and pruned) source."""
+    nx, ny, nc = datacube._array.shape[:3]
+    ix, iy, fx, fy, h, r_kind, gauss = kernel_fields(sph_kernel, source.pixcoords, nx, ny)
+    sc = source.skycoords
+    k0, fu, s, amp, c_lo, c_hi = spectral_fields(
+        sc.radial_velocity, spectral_model._sigma_kms(source), source.mHI_g, sc.distance,
+        datacube, spectral_model._mi_kind, out_scale)
+    rec = make_records(source.npart, ix, iy, fx, fy, h, r_kind, k0, fu, s, amp, c_lo, c_hi)
+    desc = _lib.make_desc(nx, ny, nc, spectral_model._mi_kind, gauss)
+    return rec, desc
+
+
+def pack_spectral(source, spectral_model, datacube):
+    """Records carrying only spectral fields (for init_spectra)."""
+    nc = datacube.n_channels
+    sc = source.skycoords
+    k0, fu, s, amp, c_lo, c_hi = spectral_fields(
+        sc.radial_velocity, spectral_model._sigma_kms(source), source.mHI_g, sc.distance,
+        datacube, spectral_model._mi_kind, 1.0)
+    n = source.npart
+    z = np.zeros(n)
+    rec = make_records(n, z, z, z, z, z, np.full(n, -256, np.int64), k0, fu, s, amp, c_lo, c_hi)
+    return rec, _lib.make_desc(1, 1, nc, spectral_model._mi_kind)

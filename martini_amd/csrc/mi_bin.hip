@@ -1,0 +1,255 @@
+// On-GPU binning of particles into (pixel tile x channel block) cells.
+//
+// Replaces the reference's per-pixel O(N) square mask
+// (martini/martini.py:271-274): every particle is listed once in each cell
+// its footprint and channel window touch, and the list is sorted by
+// (cell, kernel group, pixel Morton code) with a stable radix sort, so that
+// each cell's particles form one contiguous, spatially ordered, deterministic
+// segment.  Long segments are cut into work items of at most ITEM_MAX
+// particles; such cells get one partial tile per item.
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "mi_device.h"
+#include "mi_internal.h"
+
+namespace mi {
+
+Grid make_grid(const mi_cube_desc& d) {
+    Grid g;
+    g.nx = d.nx;
+    g.ny = d.ny;
+    g.nc = d.nc;
+    g.ntx = (d.nx + TILE - 1) / TILE;
+    g.nty = (d.ny + TILE - 1) / TILE;
+    g.ncb = (d.nc + CBLOCK - 1) / CBLOCK;
+    g.ncells = (int64_t)g.ntx * g.nty * g.ncb;
+    int b = 1;
+    while (((int64_t)1 << b) < g.ncells) ++b;
+    g.cell_bits = b;
+    return g;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = off;
+        off = align256(off + bytes);
+        return at;
+    };
+    size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)(n_pairs > 0 ? n_pairs : 1);
+    size_t nc1 = (size_t)g.ncells + 1;
+    L->items_max = g.ncells + n_pairs / ITEM_MAX + 1;
+    L->multi_max = 2 * (n_pairs / ITEM_MAX) + 1;
+    L->counts = take(4 * nn);
+    L->offsets = take(4 * nn);
+    L->keys = take(4 * np);
+    L->vals = take(4 * np);
+    L->keys2 = take(4 * np);
+    L->vals2 = take(4 * np);
+    L->cell_start = take(4 * (size_t)g.ncells);
+    L->cell_end = take(4 * (size_t)g.ncells);
+    L->item_counts = take(4 * nc1);
+    L->item_off = take(4 * nc1);
+    L->multi_counts = take(4 * nc1);
+    L->multi_off = take(4 * nc1);
+    L->items = take(sizeof(Item) * (size_t)L->items_max);
+    L->partials = take(sizeof(float) * (size_t)L->multi_max * TILE_PX * CBLOCK);
+    L->status = take(4 * 8);
+    // rocPRIM temporary storage: the largest of the three uses.
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    uint32_t* nul = nullptr;
+    hipError_t e = rocprim::exclusive_scan(nullptr, t1, nul, nul, 0u, nn,
+                                           rocprim::plus<uint32_t>(), 0);
+    if (e != hipSuccess) return e;
+    e = rocprim::radix_sort_pairs(nullptr, t2, nul, nul, nul, nul, np, 0,
+                                  g.cell_bits + KEY_LOCAL_BITS, 0);
+    if (e != hipSuccess) return e;
+    e = rocprim::exclusive_scan(nullptr, t3, nul, nul, 0u, nc1, rocprim::plus<uint32_t>(), 0);
+    if (e != hipSuccess) return e;
+    size_t tb = t1 > t2 ? t1 : t2;
+    tb = tb > t3 ? tb : t3;
+    L->temp_bytes = tb;
+    L->temp = take(tb);
+    L->total = off;
+    return hipSuccess;
+}
+
+__device__ inline uint32_t spread4(uint32_t v) {
+    v &= 15u;
+    return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2) | ((v & 8u) << 3);
+}
+
+__device__ inline uint32_t kernel_group(int kind) {
+    return kind == MI_KERNEL_GAUSS ? 1u : (kind == MI_KERNEL_DIRAC ? 2u : 0u);
+}
+
+__global__ void k_count(const mi_particle* __restrict__ P, int64_t n, Grid g,
+                        uint32_t* __restrict__ counts) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    mi_particle p = P[k];
+    int a, b, c, d, e, f;
+    counts[k] = cells_of(p, g, a, b, c, d, e, f);
+}
+
+// Writes the (key, particle) pairs of each particle at its scanned offset.
+// Entries past the device total up to the capacity get key 0xffffffff so the
+// sort moves them to the end; a total above the capacity raises the overflow
+// flag and writes nothing.
+__global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap, Grid g,
+                       const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
+                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                       uint32_t* __restrict__ status) {
+    const int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    uint64_t total = n > 0 ? (uint64_t)offsets[n - 1] + counts[n - 1] : 0;
+    if (k0 == 0) {
+        status[ST_PAIRS] = (uint32_t)total;
+        status[ST_OVERFLOW] = total > (uint64_t)cap ? 1u : 0u;
+    }
+    if (total > (uint64_t)cap) return;
+    for (int64_t q = (int64_t)total + k0; q < cap; q += stride) {  // padding entries
+        keys[q] = 0xffffffffu;
+        vals[q] = 0u;
+    }
+    for (int64_t k = k0; k < n; k += stride) {
+        mi_particle p = P[k];
+        int tx0, tx1, ty0, ty1, cb0, cb1;
+        if (cells_of(p, g, tx0, tx1, ty0, ty1, cb0, cb1) == 0) continue;
+        uint32_t at = offsets[k];
+        uint32_t kg = kernel_group(kind_of(p.r_kind));
+        for (int tx = tx0; tx <= tx1; ++tx) {
+            int lx = p.ix - tx * TILE;
+            lx = lx < 0 ? 0 : (lx > TILE - 1 ? TILE - 1 : lx);
+            for (int ty = ty0; ty <= ty1; ++ty) {
+                int ly = p.iy - ty * TILE;
+                ly = ly < 0 ? 0 : (ly > TILE - 1 ? TILE - 1 : ly);
+                uint32_t local = (kg << 8) | spread4((uint32_t)lx) | (spread4((uint32_t)ly) << 1);
+                for (int cb = cb0; cb <= cb1; ++cb) {
+                    uint32_t cell = (uint32_t)(((int64_t)tx * g.nty + ty) * g.ncb + cb);
+                    keys[at] = (cell << KEY_LOCAL_BITS) | local;
+                    vals[at] = (uint32_t)k;
+                    ++at;
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = v;
+}
+
+__global__ void k_cell_bounds(const uint32_t* __restrict__ keys, int64_t np, int64_t ncells,
+                              uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    uint32_t c = keys[q] >> KEY_LOCAL_BITS;
+    if ((int64_t)c >= ncells) return;  // padding
+    uint32_t prev = q > 0 ? keys[q - 1] >> KEY_LOCAL_BITS : 0xffffffffu;
+    uint32_t next = q + 1 < np ? keys[q + 1] >> KEY_LOCAL_BITS : 0xffffffffu;
+    if (c != prev) start[c] = (uint32_t)q;
+    if (c != next) end[c] = (uint32_t)(q + 1);
+}
+
+__global__ void k_item_counts(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                              int64_t ncells, uint32_t* __restrict__ item_counts,
+                              uint32_t* __restrict__ multi_counts) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncells) return;
+    uint32_t ni = 0;
+    if (c < ncells) {
+        uint32_t len = end[c] - start[c];
+        ni = (len + ITEM_MAX - 1) / ITEM_MAX;
+    }
+    item_counts[c] = ni;
+    multi_counts[c] = ni > 1 ? ni : 0;
+}
+
+__global__ void k_build_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
+                              int64_t ncells, const uint32_t* __restrict__ item_off,
+                              const uint32_t* __restrict__ multi_off, Item* __restrict__ items,
+                              uint32_t* __restrict__ status) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) {
+        status[ST_ITEMS] = item_off[ncells];
+        status[ST_MULTI] = multi_off[ncells];
+    }
+    if (c >= ncells) return;
+    uint32_t s0 = start[c], e0 = end[c];
+    uint32_t ni = (e0 - s0 + ITEM_MAX - 1) / ITEM_MAX;
+    for (uint32_t q = 0; q < ni; ++q) {
+        Item it;
+        it.cell = (int32_t)c;
+        it.begin = (int32_t)(s0 + q * ITEM_MAX);
+        it.end = (int32_t)min(e0, s0 + (q + 1) * ITEM_MAX);
+        it.slot = ni > 1 ? (int32_t)(multi_off[c] + q) : -1;
+        items[item_off[c] + q] = it;
+    }
+}
+
+static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
+                         char* ws, const Layout& L, hipStream_t s) {
+    uint32_t* counts = (uint32_t*)(ws + L.counts);
+    uint32_t* offsets = (uint32_t*)(ws + L.offsets);
+    uint32_t* keys = (uint32_t*)(ws + L.keys);
+    uint32_t* vals = (uint32_t*)(ws + L.vals);
+    uint32_t* keys2 = (uint32_t*)(ws + L.keys2);
+    uint32_t* vals2 = (uint32_t*)(ws + L.vals2);
+    uint32_t* cstart = (uint32_t*)(ws + L.cell_start);
+    uint32_t* cend = (uint32_t*)(ws + L.cell_end);
+    uint32_t* icount = (uint32_t*)(ws + L.item_counts);
+    uint32_t* ioff = (uint32_t*)(ws + L.item_off);
+    uint32_t* mcount = (uint32_t*)(ws + L.multi_counts);
+    uint32_t* moff = (uint32_t*)(ws + L.multi_off);
+    Item* items = (Item*)(ws + L.items);
+    uint32_t* status = (uint32_t*)(ws + L.status);
+    void* temp = ws + L.temp;
+    size_t tb = L.temp_bytes;
+    const int T = 256;
+
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(T), 0, s, status, (int64_t)8, 0u);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_count, dim3(blocks_for(n, T)), dim3(T), 0, s, d_p, n, g, counts);
+        hipError_t e = rocprim::exclusive_scan(temp, tb, counts, offsets, 0u, (size_t)n,
+                                               rocprim::plus<uint32_t>(), s);
+        if (e != hipSuccess) return e;
+    }
+    int64_t emit_threads = n > n_pairs ? n : n_pairs;
+    if (emit_threads < 1) emit_threads = 1;
+    unsigned eb = blocks_for(emit_threads, T);
+    if (eb > 65536u) eb = 65536u;
+    hipLaunchKernelGGL(k_emit, dim3(eb), dim3(T), 0, s, d_p, n, n_pairs, g, counts, offsets, keys,
+                       vals, status);
+    if (n_pairs > 0) {
+        hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys2, vals, vals2,
+                                                 (size_t)n_pairs, 0,
+                                                 g.cell_bits + KEY_LOCAL_BITS, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cstart, g.ncells, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cend, g.ncells, 0u);
+    if (n_pairs > 0)
+        hipLaunchKernelGGL(k_cell_bounds, dim3(blocks_for(n_pairs, T)), dim3(T), 0, s, keys2,
+                           n_pairs, g.ncells, cstart, cend);
+    hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(g.ncells + 1, T)), dim3(T), 0, s, cstart,
+                       cend, g.ncells, icount, mcount);
+    hipError_t e = rocprim::exclusive_scan(temp, tb, icount, ioff, 0u, (size_t)g.ncells + 1,
+                                           rocprim::plus<uint32_t>(), s);
+    if (e != hipSuccess) return e;
+    e = rocprim::exclusive_scan(temp, tb, mcount, moff, 0u, (size_t)g.ncells + 1,
+                                rocprim::plus<uint32_t>(), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_build_items, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cstart, cend,
+                       g.ncells, ioff, moff, items, status);
+    return hipGetLastError();
+}
+
+}  // namespace mi

@@ -1,0 +1,113 @@
+// extern "C" entry points of libmartini_mi355x (declared in
+// include/martini_mi355x.h).  Argument checking, workspace layout, and the
+// stage sequence bin -> gather -> reduce.
+#include <hip/hip_runtime.h>
+
+#include "../../include/martini_mi355x.h"
+#include "mi_device.h"
+#include "mi_internal.h"
+
+using namespace mi;
+
+static bool desc_ok(const mi_cube_desc* d) {
+    if (!d) return false;
+    if (d->nx <= 0 || d->ny <= 0 || d->nc <= 0) return false;
+    if (d->spectrum != MI_SPECTRUM_GAUSS && d->spectrum != MI_SPECTRUM_DIRAC) return false;
+    return true;
+}
+
+// Limits that the key and index widths impose.
+static int check_limits(const Grid& g, int64_t n, int64_t n_pairs) {
+    if (g.ncells >= ((int64_t)1 << 21)) return MI_ERR_LIMIT;  // cell id fits 22 key bits
+    if (n >= ((int64_t)1 << 31) || n_pairs >= ((int64_t)1 << 31)) return MI_ERR_LIMIT;
+    if (g.nx > 32767 || g.ny > 32767) return MI_ERR_LIMIT;
+    return MI_OK;
+}
+
+extern "C" {
+
+int mi_abi_version(void) { return MI_ABI_VERSION; }
+
+const char* mi_strerror(int code) {
+    switch (code) {
+    case MI_OK: return "success";
+    case MI_ERR_ARGUMENT: return "invalid argument";
+    case MI_ERR_WORKSPACE: return "workspace too small for this input";
+    case MI_ERR_DEVICE: return "HIP runtime error";
+    case MI_ERR_UNSUPPORTED: return "unsupported kernel, spectrum or method";
+    case MI_ERR_LIMIT: return "input exceeds a size limit of the library";
+    }
+    return "unknown error";
+}
+
+int64_t mi_count_pairs_host(const mi_particle* h_particles, int64_t n, const mi_cube_desc* desc) {
+    if (!desc_ok(desc) || n < 0 || (n > 0 && !h_particles)) return MI_ERR_ARGUMENT;
+    Grid g = make_grid(*desc);
+    int64_t tot = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        int a, b, c, d, e, f;
+        tot += cells_of(h_particles[k], g, a, b, c, d, e, f);
+    }
+    return tot;
+}
+
+int mi_insert_workspace_size(const mi_cube_desc* desc, int64_t n, int64_t n_pairs,
+                             size_t* bytes) {
+    if (!desc_ok(desc) || n < 0 || n_pairs < 0 || !bytes) return MI_ERR_ARGUMENT;
+    Grid g = make_grid(*desc);
+    int rc = check_limits(g, n, n_pairs);
+    if (rc) return rc;
+    Layout L;
+    if (layout_for(g, n, n_pairs, &L) != hipSuccess) return MI_ERR_DEVICE;
+    *bytes = L.total;
+    return MI_OK;
+}
+
+int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+              const mi_cube_desc* desc, float* d_cube, void* d_workspace, size_t workspace_bytes,
+              int method, void* stream) {
+    if (!desc_ok(desc) || n < 0 || n_pairs < 0 || !d_cube || !d_workspace) return MI_ERR_ARGUMENT;
+    if (n > 0 && !d_particles) return MI_ERR_ARGUMENT;
+    if (method != MI_METHOD_PLAIN_F32 && method != MI_METHOD_MFMA_BF16X3) return MI_ERR_UNSUPPORTED;
+    Grid g = make_grid(*desc);
+    int rc = check_limits(g, n, n_pairs);
+    if (rc) return rc;
+    Layout L;
+    if (layout_for(g, n, n_pairs, &L) != hipSuccess) return MI_ERR_DEVICE;
+    if (workspace_bytes < L.total) return MI_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    char* ws = (char*)d_workspace;
+    if (bin_particles(d_particles, n, n_pairs, g, ws, L, s) != hipSuccess) return MI_ERR_DEVICE;
+    hipError_t e = method == MI_METHOD_PLAIN_F32 ? gather_plain(d_particles, g, *desc, ws, L, d_cube, s)
+                                                 : gather_mfma(d_particles, g, *desc, ws, L, d_cube, s);
+    if (e == hipErrorNotSupported) return MI_ERR_UNSUPPORTED;
+    if (e != hipSuccess) return MI_ERR_DEVICE;
+    if (reduce_partials(g, ws, L, d_cube, s) != hipSuccess) return MI_ERR_DEVICE;
+    uint32_t st[8];
+    if (hipMemcpyAsync(st, ws + L.status, sizeof(st), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return MI_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return MI_ERR_DEVICE;
+    if (st[ST_OVERFLOW]) return MI_ERR_WORKSPACE;
+    return MI_OK;
+}
+
+int mi_kernel_weights(int kind, float gauss_t, float gauss_inv_norm, const float* d_dx,
+                      const float* d_dy, const float* d_h, float* d_out, int64_t n,
+                      void* stream) {
+    if (n < 0 || (n > 0 && (!d_dx || !d_dy || !d_h || !d_out))) return MI_ERR_ARGUMENT;
+    if (kind < MI_KERNEL_CUBIC || kind > MI_KERNEL_DIRAC) return MI_ERR_UNSUPPORTED;
+    if (launch_weights(kind, gauss_t, gauss_inv_norm, d_dx, d_dy, d_h, d_out, n,
+                       (hipStream_t)stream) != hipSuccess)
+        return MI_ERR_DEVICE;
+    return MI_OK;
+}
+
+int mi_spectra(const mi_particle* d_particles, int64_t n, const mi_cube_desc* desc, float* d_out,
+               void* stream) {
+    if (!desc_ok(desc) || n < 0 || (n > 0 && (!d_particles || !d_out))) return MI_ERR_ARGUMENT;
+    if (launch_spectra(d_particles, n, *desc, d_out, (hipStream_t)stream) != hipSuccess)
+        return MI_ERR_DEVICE;
+    return MI_OK;
+}
+
+}  // extern "C"

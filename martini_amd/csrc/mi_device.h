@@ -1,0 +1,191 @@
+// Device math shared by the insertion kernels: SPH kernel integrals over a
+// pixel and the per-channel line profile.
+//
+// Kernel integrals restate martini/sph_kernels.py (cubic :791-858,
+// WendlandC2 :400-441, WendlandC6 :544-685, quartic :1486-1564, Gaussian
+// :994-1044, Dirac :1136-1165).  The polynomial kernels are evaluated in FP64:
+// their closed forms sum terms ~30x larger than the result, which costs FP32
+// about 1e-6 of the particle's peak weight.  The Gaussian kernel and the
+// spectra use FP32 erf/erfc with the difference of two erf values taken on
+// the side where both are small, so tails keep their relative precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/martini_mi355x.h"
+
+namespace mi {
+
+// 16 x 16 pixel tiles, 64-channel blocks: one cell per (tile, block).
+constexpr int TILE = 16;
+constexpr int TILE_PX = TILE * TILE;
+constexpr int CBLOCK = 64;
+// Particles per cell processed by one work item (longer cells are split and
+// their partial tiles summed in item order by a reduction pass).
+constexpr int ITEM_MAX = 4096;
+// Key layout: cell << KEY_LOCAL_BITS | kernel group << 8 | 8-bit pixel Morton code.
+constexpr int KEY_LOCAL_BITS = 10;
+
+__host__ __device__ inline int kind_of(int32_t r_kind) { return r_kind & 15; }
+__host__ __device__ inline int gidx_of(int32_t r_kind) { return (r_kind >> 4) & 3; }
+__host__ __device__ inline int range_of(int32_t r_kind) { return r_kind >> 8; }
+
+// Clipped footprint of a particle: pixel rows [i_lo, i_hi] and columns
+// [j_lo, j_hi] with |i - x| <= r (martini/martini.py:271-274).  Returns false
+// when empty.  x = ix + fx with fx in [0, 1), r an integer, so
+// ceil(x - r) = ix - r + (fx > 0) and floor(x + r) = ix + r exactly.
+__host__ __device__ inline bool footprint(const mi_particle& p, int nx, int ny, int& i_lo,
+                                          int& i_hi, int& j_lo, int& j_hi) {
+    int r = range_of(p.r_kind);
+    if (r < 0) return false;
+    i_lo = p.ix - r + (p.fx > 0.0f ? 1 : 0);
+    i_hi = p.ix + r;
+    j_lo = p.iy - r + (p.fy > 0.0f ? 1 : 0);
+    j_hi = p.iy + r;
+    if (i_lo < 0) i_lo = 0;
+    if (j_lo < 0) j_lo = 0;
+    if (i_hi > nx - 1) i_hi = nx - 1;
+    if (j_hi > ny - 1) j_hi = ny - 1;
+    return i_lo <= i_hi && j_lo <= j_hi;
+}
+
+// ---------------------------------------------------------------- kernels
+__device__ inline double w_cubic(double dx, double dy, double h) {
+    double R2 = 4.0 * (dx * dx + dy * dy) / (h * h), F;
+    if (R2 == 0.0) {
+        F = 0.75;
+    } else if (R2 <= 1.0) {
+        double A = sqrt(1.0 - R2), B = sqrt(4.0 - R2);
+        double lA = log((1.0 + A) / sqrt(R2));
+        double I1 = A - 0.5 * A * A * A - 1.5 * R2 * A + (3.0 / 32.0) * A * (3.0 * R2 + 2.0) +
+                    (9.0 / 32.0) * R2 * R2 * lA;
+        double I3 = -B * (3.0 * R2 + 56.0) * 0.25 + A * (4.0 * R2 + 50.0) * 0.125 -
+                    0.375 * R2 * (R2 + 16.0) * log((2.0 + B) / (1.0 + A)) +
+                    2.0 * (3.0 * R2 + 4.0) * (B - A) + 2.0 * (B * B * B - A * A * A);
+        F = I1 + 0.25 * I3;
+    } else if (R2 <= 4.0) {
+        double B = sqrt(4.0 - R2);
+        double I2 = -B * (3.0 * R2 + 56.0) * 0.25 -
+                    0.375 * R2 * (R2 + 16.0) * log((2.0 + B) / sqrt(R2)) +
+                    2.0 * (3.0 * R2 + 4.0) * B + 2.0 * B * B * B;
+        F = 0.25 * I2;
+    } else {
+        return 0.0;
+    }
+    return F * (4.0 / 1.59689476201133) / (h * h);
+}
+
+__device__ inline double w_wc2(double dx, double dy, double h) {
+    double R2 = (dx * dx + dy * dy) / (h * h), G;
+    if (R2 == 0.0) {
+        G = 2.0 / 3.0;
+    } else if (R2 < 1.0) {
+        double A = sqrt(1.0 - R2);
+        G = 5.0 * R2 * R2 * (0.5 * R2 + 3.0) * log((1.0 + A) / sqrt(R2)) +
+            A * (-13.5 * R2 * R2 - (14.0 / 3.0) * R2 + 2.0 / 3.0);
+    } else {
+        return 0.0;
+    }
+    return G * (21.0 / (2.0 * M_PI)) / (h * h);
+}
+
+// WendlandC6 indefinite integral in z at fixed R (the reference's rounded
+// coefficients), grouped by the factor each term carries.
+__device__ inline double wc6_indef(double R, double z) {
+    double R2 = R * R, z2 = z * z;
+    double S = sqrt(R2 + z2), L = log(S + z);
+    double R4 = R2 * R2, R6 = R4 * R2, R8 = R4 * R4, R10 = R8 * R2, R12 = R6 * R6;
+    double z3 = z2 * z, z5 = z3 * z2, z7 = z5 * z2, z9 = z7 * z2, z11 = z9 * z2;
+    double plain = -231.0 * R10 * z - 385.0 * R8 * z3 - 1155.0 * R8 * z - 462.0 * R6 * z5 -
+                   1540.0 * R6 * z3 - 462.0 * R6 * z - 330.0 * R4 * z7 - 1386.0 * R4 * z5 -
+                   462.0 * R4 * z3 + 66.0 * R4 * z - (128.0 + 1.0 / 3.0) * R2 * z9 -
+                   660.0 * R2 * z7 - 277.2 * R2 * z5 + 44.0 * R2 * z3 - 11.0 * R2 * z -
+                   21.0 * z11 - (128.0 + 1.0 / 3.0) * z9 - 66.0 * z7 + 13.2 * z5 -
+                   (11.0 / 3.0) * z3 + z;
+    double withS = (8.0 / 3.0) * z11 + (16.0 + 4.0 / 15.0) * R2 * z9 + 70.4 * z9 +
+                   360.8 * R2 * z7 + 132.0 * z7 + 550.0 * R2 * z5 + 24.7813 * R10 * z +
+                   530.75 * R8 * z + 47.4792 * R8 * z3 + 767.25 * R6 * z + 58.0167 * R6 * z5 +
+                   819.5 * R6 * z3 + 41.7 * R4 * z7 + 752.4 * R4 * z5 + 896.5 * R4 * z3;
+    double withL = 7.21875 * R12 + 173.25 * R10 + 288.75 * R8;
+    return plain + withS * S + withL * L;
+}
+
+__device__ inline double w_wc6(double dx, double dy, double h) {
+    double R = sqrt(dx * dx + dy * dy) / h, v;
+    const double norm = 1365.0 / (64.0 * M_PI);
+    if (R == 0.0) v = norm * 2.0 * (4.0 / 15.0);
+    else if (R < 1.0) v = norm * 2.0 * (wc6_indef(R, sqrt(1.0 - R * R)) - wc6_indef(R, 0.0));
+    else return 0.0;
+    return v / (h * h);
+}
+
+// Indefinite integral over z of (A - sqrt(R^2+z^2))^4 at fixed R.
+__device__ inline double quartic_IA(double R, double z, double A) {
+    double q = sqrt(z * z + R * R), R2 = R * R, A2 = A * A;
+    return A2 * A2 * z - 2.0 * A2 * A * z * q + 2.0 * A2 * z * (3.0 * R2 + z * z) -
+           0.5 * A * R2 * (4.0 * A2 + 3.0 * R2) * asinh(z / R) -
+           0.5 * A * z * q * (5.0 * R2 + 2.0 * z * z) + R2 * R2 * z + (2.0 / 3.0) * R2 * z * z * z +
+           0.2 * z * z * z * z * z;
+}
+
+__device__ inline double w_quartic(double dx, double dy, double h) {
+    double R = sqrt(dx * dx + dy * dy) / h, v = 0.0;
+    if (R == 0.0) {
+        v = 384.0 / 3125.0;
+    } else if (R < 1.0) {
+        if (R < 0.2) v += 10.0 * quartic_IA(R, sqrt(0.04 - R * R), 0.2);
+        if (R < 0.6) v -= 5.0 * quartic_IA(R, sqrt(0.36 - R * R), 0.6);
+        v += quartic_IA(R, sqrt(1.0 - R * R), 1.0);
+    } else {
+        return 0.0;
+    }
+    return v * (2.0 * 15625.0 / (512.0 * M_PI)) / (h * h);
+}
+
+// erf(b) - erf(a) for a <= b, taken where the two terms do not cancel.
+__device__ inline float erf_diff(float a, float b) {
+    if (a >= 0.0f) return erfcf(a) - erfcf(b);
+    if (b <= 0.0f) return erfcf(-b) - erfcf(-a);
+    return erff(b) - erff(a);
+}
+
+constexpr float GAUSS_SIG = 0.42466090014400953f;  // 1 / (2 sqrt(2 ln 2)): FWHM = 1
+
+__device__ inline float w_gauss(float dx, float dy, float h, float t, float inv_norm) {
+    float dr = sqrtf(dx * dx + dy * dy);
+    float hs = h * GAUSS_SIG;
+    float u = dr / hs;
+    if ((dr - 0.70710678118654752f) / hs > t) return 0.0f;  // sph_kernels.py:1041
+    float zmax = (t > u) ? sqrtf(t * t - u * u) : 0.0f;
+    float c = 1.0f / (hs * 1.41421356237309505f);
+    float ex = erf_diff((dx - 0.5f) * c, (dx + 0.5f) * c);
+    float ey = erf_diff((dy - 0.5f) * c, (dy + 0.5f) * c);
+    return 0.25f * erff(zmax * 0.70710678118654752f) * ex * ey * inv_norm;
+}
+
+// Weight of particle p at pixel (i, j): integral of its kernel over the pixel
+// per unit pixel area (sph_kernels.py:85-119 with h = h_px * rescale).
+__device__ inline float pixel_weight(int kind, float dx, float dy, float h, float gt, float ginv) {
+    switch (kind) {
+    case MI_KERNEL_CUBIC: return (float)w_cubic((double)dx, (double)dy, (double)h);
+    case MI_KERNEL_WC2: return (float)w_wc2((double)dx, (double)dy, (double)h);
+    case MI_KERNEL_WC6: return (float)w_wc6((double)dx, (double)dy, (double)h);
+    case MI_KERNEL_QUARTIC: return (float)w_quartic((double)dx, (double)dy, (double)h);
+    case MI_KERNEL_GAUSS: return w_gauss(dx, dy, h, gt, ginv);
+    case MI_KERNEL_DIRAC: return (fabsf(dx) < 0.5f && fabsf(dy) < 0.5f) ? 1.0f : 0.0f;
+    }
+    return 0.0f;
+}
+
+// Spectrum of particle p in channel c (Jy per unit weight): amp times the
+// Gaussian line integrated over the channel (spectral_models.py:347-425), or
+// amp inside the Dirac window (:510-570).
+__device__ inline float channel_flux(const mi_particle& p, int c, int spectrum) {
+    if (c < p.c_lo || c > p.c_hi) return 0.0f;
+    if (spectrum == MI_SPECTRUM_DIRAC) return p.amp;
+    float d = (float)(p.k0 - c) + p.fu;  // u - c
+    return 0.5f * p.amp * erf_diff(p.s * (d - 1.0f), p.s * d);
+}
+
+}  // namespace mi

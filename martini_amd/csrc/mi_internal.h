@@ -1,0 +1,85 @@
+// Internal interfaces between the library's translation units: workspace
+// layout and the host-side launchers of each stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/martini_mi355x.h"
+#include "mi_device.h"
+
+namespace mi {
+
+struct Grid {
+    int nx, ny, nc;      // padded cube extent
+    int ntx, nty, ncb;   // tiles along x, y; channel blocks
+    int64_t ncells;
+    int cell_bits;       // bits needed for a cell id
+};
+
+Grid make_grid(const mi_cube_desc& d);
+
+// Number of cells particle p lands in (0 if its footprint or window is empty).
+__host__ __device__ inline uint32_t cells_of(const mi_particle& p, const Grid& g, int& tx0, int& tx1,
+                                    int& ty0, int& ty1, int& cb0, int& cb1) {
+    int i_lo, i_hi, j_lo, j_hi;
+    if (!footprint(p, g.nx, g.ny, i_lo, i_hi, j_lo, j_hi)) return 0;
+    int c_lo = p.c_lo < 0 ? 0 : p.c_lo;
+    int c_hi = p.c_hi > g.nc - 1 ? g.nc - 1 : p.c_hi;
+    if (c_lo > c_hi) return 0;
+    tx0 = i_lo / TILE;
+    tx1 = i_hi / TILE;
+    ty0 = j_lo / TILE;
+    ty1 = j_hi / TILE;
+    cb0 = c_lo / CBLOCK;
+    cb1 = c_hi / CBLOCK;
+    return (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1) * (cb1 - cb0 + 1));
+}
+
+
+// Byte offsets of the workspace regions (each 256-byte aligned).
+struct Layout {
+    size_t counts, offsets;            // uint32[n] each
+    size_t keys, vals, keys2, vals2;   // uint32[n_pairs] each
+    size_t cell_start, cell_end;       // uint32[ncells] each
+    size_t item_counts, item_off;      // uint32[ncells + 1] each
+    size_t multi_counts, multi_off;    // uint32[ncells + 1] each
+    size_t items;                      // int4[items_max]
+    size_t partials;                   // float[multi_max * TILE_PX * CBLOCK]
+    size_t status;                     // uint32[8]
+    size_t temp;                       // rocPRIM temporary storage
+    size_t temp_bytes;
+    size_t total;
+    int64_t items_max, multi_max;
+};
+
+// status[0]: number of pairs the device produced; status[1]: overflow flag.
+enum { ST_PAIRS = 0, ST_OVERFLOW = 1, ST_ITEMS = 2, ST_MULTI = 3 };
+
+hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L);
+
+// Binning: count, scan, emit, sort, cell bounds, items.  Asynchronous.
+hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
+                         char* ws, const Layout& L, hipStream_t s);
+
+// Gather over work items (plain FP32 or MFMA), then the partial-tile reduction.
+hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
+                        const Layout& L, float* d_cube, hipStream_t s);
+hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
+                       const Layout& L, float* d_cube, hipStream_t s);
+hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cube,
+                           hipStream_t s);
+
+hipError_t launch_weights(int kind, float gt, float ginv, const float* dx, const float* dy,
+                          const float* h, float* out, int64_t n, hipStream_t s);
+hipError_t launch_spectra(const mi_particle* d_p, int64_t n, const mi_cube_desc& d, float* out,
+                          hipStream_t s);
+
+// One work item: particles [begin, end) of the sorted pair list belonging to
+// one cell; slot >= 0 names its partial tile, -1 means it writes the cube.
+struct Item {
+    int32_t cell, begin, end, slot;
+};
+
+}  // namespace mi

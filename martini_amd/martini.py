@@ -1,0 +1,149 @@
+"""Martini: the mock-observation orchestrator, insertion path only.
+
+Mirrors martini.martini.Martini / _BaseMartini (martini/martini.py:47-425,
+:627-861): the constructor runs the same FP64 setup sequence (sky and pixel
+coordinates, smoothing lengths, ranges, pruning; :118-151, :168-241), and
+``insert_source_in_cube`` keeps its signature and effect (:827-861): the
+cube array is accumulated (+=) and converted to Jy/arcsec^2, and a flux and
+mass summary is printed unless quiet.
+
+The pixel loop of the reference (:243-283, :327-362) is replaced by the GPU
+path: records packed on the host (``_pack``), then binning, a tile gather
+(``method='plain'`` FP32 or ``'mfma'`` BF16x3) and a partial-tile reduction
+on the device.  ``ncpu`` and ``progressbar`` are accepted for compatibility
+and have no effect.  Beam, noise and output writers are not part of this
+package: passing a beam or noise raises NotImplementedError.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _pack
+from .insertion import default_inserter
+
+
+class _BaseMartini:
+    def __init__(self, *, source, datacube, beam=None, noise=None, sph_kernel, spectral_model,
+                 quiet: bool = False, _prune_kwargs: dict | None = None) -> None:
+        if beam is not None:
+            raise NotImplementedError("beam convolution is not part of martini_amd")
+        if noise is not None:
+            raise NotImplementedError("noise is not part of martini_amd")
+        self.quiet = quiet
+        self.source = source
+        self._datacube = datacube
+        self.beam = None
+        self.noise = None
+        self.sph_kernel = sph_kernel
+        self.spectral_model = spectral_model
+        self.source._init_skycoords()
+        self.source._init_pixcoords(self._datacube)
+        self.sph_kernel._init_sm_lengths(source=self.source, datacube=self._datacube)
+        self.sph_kernel._init_sm_ranges()
+        self._prune_particles(**(_prune_kwargs or {}))
+
+    def init_spectra(self) -> None:
+        """N x Nc spectra, for callers that want them (martini.py:153-166)."""
+        if not self.quiet:
+            print("Initializing spectra...")
+        self.spectral_model.init_spectra(self.source, self._datacube)
+        if not self.quiet:
+            print("Spectra initialized.")
+
+    def _prune_particles(self, spatial: bool = True, spectral: bool = True, mass: bool = True,
+                         obj_type_str: str = "data cube") -> None:
+        """Drop particles that cannot contribute (martini.py:168-241)."""
+        src, dc = self.source, self._datacube
+        if not self.quiet:
+            print(f"Source module contained {src.npart} particles with total HI mass of "
+                  f"{src.mHI_g.sum():.2e} solMass.")
+        x, y, z = src.pixcoords
+        r = np.asarray(self.sph_kernel.sm_ranges, float)
+        dv_max = np.max(np.abs(np.diff(dc.velocity_channel_edges))) / 1e3  # km/s
+        hw = np.asarray(self.spectral_model.half_width(src), float) / dv_max
+        reject = np.zeros(src.npart, bool)
+        with np.errstate(invalid="ignore"):
+            if spatial:
+                reject |= (x + r < 0) | (y + r < 0)
+                reject |= x - r > dc.n_px_x + dc.padx * 2
+                reject |= y - r > dc.n_px_y + dc.pady * 2
+                reject |= np.isnan(x) | np.isnan(y)
+            if spectral:
+                reject |= z + 4 * hw < 0
+                reject |= z - 4 * hw > dc.n_channels
+        if mass:
+            reject |= src.mHI_g == 0
+        keep = ~reject
+        src.apply_mask(keep)
+        self.sph_kernel._apply_mask(keep)
+        if not self.quiet:
+            print(f"Pruned particles that will not contribute to {obj_type_str}, {src.npart} "
+                  f"particles remaining with total HI mass of {src.mHI_g.sum():.2e} solMass.")
+
+    def _insert_source_in_cube(self, skip_validation: bool = False, progressbar=None,
+                               ncpu: int = 1, quiet=None, method: str = "plain") -> None:
+        """Fill the cube on the GPU (replaces martini.py:285-407)."""
+        self.sph_kernel._confirm_validation(noraise=skip_validation, quiet=self.quiet)
+        dc = self._datacube
+        px2 = dc.px_size ** 2
+        rec, desc = _pack.pack_particles(self.source, self.sph_kernel, self.spectral_model, dc,
+                                         out_scale=1.0 / px2)
+        d_cube = default_inserter().insert(rec, desc, method=method)
+        added = d_cube.cpu().numpy().astype(np.float64).reshape(dc._array.shape[:3])
+        if dc._array_unit == "Jy/pix2":
+            dc._array = dc._array / px2
+            dc._array_unit = "Jy/arcsec2"
+        if dc.stokes_axis:
+            dc._array[..., 0] += added
+        else:
+            dc._array += added
+        self._report(quiet)
+
+    def _report(self, quiet) -> None:
+        if not ((quiet is None and not self.quiet) or (quiet is not None and not quiet)):
+            return
+        dc = self._datacube
+        pad = (np.s_[dc.padx:-dc.padx, dc.pady:-dc.pady, ...] if dc.padx > 0 and dc.pady > 0
+               else np.s_[...])
+        flux = np.sum(dc._array[pad] * dc.px_size ** 2)
+        dv = np.abs(np.diff(dc.velocity_channel_edges)) / 1e3
+        per_chan = (dc._array[pad] * dc.px_size ** 2).sum((0, 1)).squeeze()
+        mass = 2.36e5 * self.source.distance ** 2 * np.sum(per_chan * dv)
+        nz = dc._array[dc._array > 0]
+        print("Source inserted.",
+              f"  Flux density in cube: {flux:.2e} Jy",
+              f"  Mass in cube (assuming distance {self.source.distance:.2f} Mpc and a spatially "
+              f"resolved source): {mass:.2e} solMass",
+              f"    [{mass / self.source.input_mass * 100:.0f}% of initial source mass]",
+              f"  Maximum pixel: {dc._array.max():.2e} Jy / arcsec2",
+              f"  Median non-zero pixel: {np.median(nz) if nz.size else 0.0:.2e} Jy / arcsec2",
+              sep="\n")
+
+    def reset(self) -> None:
+        """Zero the cube, keeping its geometry and padding (martini.py:409-425)."""
+        dc = self._datacube
+        dc._array = np.zeros(dc._shape())
+        dc._array_unit = "Jy/pix2"
+
+
+class Martini(_BaseMartini):
+    """Creates synthetic HI data cubes from SPH particles (martini.py:627-861)."""
+
+    def __init__(self, *, source, datacube, beam=None, noise=None, sph_kernel, spectral_model,
+                 quiet: bool = False) -> None:
+        super().__init__(source=source, datacube=datacube, beam=beam, noise=noise,
+                         sph_kernel=sph_kernel, spectral_model=spectral_model, quiet=quiet)
+
+    @property
+    def datacube(self):
+        return self._datacube
+
+    def insert_source_in_cube(self, skip_validation: bool = False, progressbar=None,
+                              ncpu: int = 1, method: str = "plain") -> None:
+        """Populate the cube with the source's flux (martini.py:827-861).
+
+        ``method``: ``'plain'`` (FP32 tile gather) or ``'mfma'`` (BF16x3 MFMA).
+        """
+        self._insert_source_in_cube(skip_validation=skip_validation, progressbar=progressbar,
+                                    ncpu=ncpu, method=method)
