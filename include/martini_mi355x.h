@@ -129,6 +129,30 @@ int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
               size_t workspace_bytes, int method, void* stream);
 
 /*
+ * The two stages of mi_insert, asynchronous (no host synchronisation), for
+ * callers that time or overlap them.  mi_bin sorts the particles into cells
+ * inside the workspace; mi_gather then evaluates the contraction over those
+ * cells with `method` and adds it to d_cube.  The workspace must not be
+ * reused between the two calls.  Overflow of n_pairs is reported by
+ * mi_status (which synchronises the stream); a gather after an overflowed
+ * bin leaves the cube untouched.
+ */
+int mi_bin(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+           const mi_cube_desc* desc, void* d_workspace, size_t workspace_bytes, void* stream);
+int mi_gather(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+              const mi_cube_desc* desc, float* d_cube, void* d_workspace,
+              size_t workspace_bytes, int method, void* stream);
+
+/*
+ * Synchronises the stream and reports the last binning in this workspace:
+ * out[0] = pairs produced, out[1] = overflow flag, out[2] = work items,
+ * out[3] = split work items (partial tiles).  Returns MI_ERR_WORKSPACE on
+ * overflow.
+ */
+int mi_status(const mi_cube_desc* desc, int64_t n, int64_t n_pairs, const void* d_workspace,
+              uint32_t out[4], void* stream);
+
+/*
  * Kernel integrals at n offsets: out[k] = W(dx[k], dy[k], h[k]) for one
  * concrete kernel kind (gaussian: truncation t, 1/norm).  Device arrays.
  * Replaces _BaseSPHKernel._px_weight / _kernel_integral

@@ -69,6 +69,14 @@ def lib() -> ctypes.CDLL:
     L.mi_insert.restype = ctypes.c_int
     L.mi_insert.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, vp, ctypes.c_size_t,
                             ctypes.c_int, vp]
+    L.mi_bin.restype = ctypes.c_int
+    L.mi_bin.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, ctypes.c_size_t, vp]
+    L.mi_gather.restype = ctypes.c_int
+    L.mi_gather.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, vp, ctypes.c_size_t,
+                            ctypes.c_int, vp]
+    L.mi_status.restype = ctypes.c_int
+    L.mi_status.argtypes = [ctypes.POINTER(CubeDesc), i64, i64, vp,
+                            ctypes.POINTER(ctypes.c_uint32 * 4), vp]
     L.mi_kernel_weights.restype = ctypes.c_int
     L.mi_kernel_weights.argtypes = [ctypes.c_int, f32, f32, vp, vp, vp, vp, i64, vp]
     L.mi_spectra.restype = ctypes.c_int
@@ -132,6 +140,32 @@ def insert(d_records, n: int, n_pairs: int, desc: CubeDesc, d_cube, d_ws, method
                          ctypes.c_void_p(d_cube.data_ptr()), ctypes.c_void_p(d_ws.data_ptr()),
                          d_ws.numel() * d_ws.element_size(), m, stream_ptr(stream))
     check(rc, "mi_insert")
+
+
+def bin(d_records, n: int, n_pairs: int, desc: CubeDesc, d_ws, stream=None) -> None:
+    """Asynchronous binning stage (mi_bin)."""
+    check(lib().mi_bin(ctypes.c_void_p(d_records.data_ptr()), n, n_pairs, ctypes.byref(desc),
+                       ctypes.c_void_p(d_ws.data_ptr()), d_ws.numel() * d_ws.element_size(),
+                       stream_ptr(stream)), "mi_bin")
+
+
+def gather(d_records, n: int, n_pairs: int, desc: CubeDesc, d_cube, d_ws, method="plain",
+           stream=None) -> None:
+    """Asynchronous gather + reduction stage (mi_gather)."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    check(lib().mi_gather(ctypes.c_void_p(d_records.data_ptr()), n, n_pairs, ctypes.byref(desc),
+                          ctypes.c_void_p(d_cube.data_ptr()), ctypes.c_void_p(d_ws.data_ptr()),
+                          d_ws.numel() * d_ws.element_size(), m, stream_ptr(stream)), "mi_gather")
+
+
+def status(desc: CubeDesc, n: int, n_pairs: int, d_ws, stream=None):
+    """(pairs, overflow, items, split items) of the last binning; synchronises."""
+    out = (ctypes.c_uint32 * 4)()
+    rc = lib().mi_status(ctypes.byref(desc), n, n_pairs, ctypes.c_void_p(d_ws.data_ptr()),
+                         ctypes.byref(out), stream_ptr(stream))
+    if rc not in (MI_OK, MI_ERR_WORKSPACE):
+        check(rc, "mi_status")
+    return tuple(int(v) for v in out)
 
 
 def kernel_weights(kind: int, dx, dy, h, gauss_t=0.0, gauss_norm=1.0) -> np.ndarray:

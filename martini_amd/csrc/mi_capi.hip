@@ -63,32 +63,83 @@ int mi_insert_workspace_size(const mi_cube_desc* desc, int64_t n, int64_t n_pair
     return MI_OK;
 }
 
-int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+static int prepare(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+                   const mi_cube_desc* desc, const void* d_workspace, size_t workspace_bytes,
+                   Grid* g, Layout* L) {
+    if (!desc_ok(desc) || n < 0 || n_pairs < 0 || !d_workspace) return MI_ERR_ARGUMENT;
+    if (n > 0 && !d_particles) return MI_ERR_ARGUMENT;
+    *g = make_grid(*desc);
+    int rc = check_limits(*g, n, n_pairs);
+    if (rc) return rc;
+    if (layout_for(*g, n, n_pairs, L) != hipSuccess) return MI_ERR_DEVICE;
+    if (workspace_bytes < L->total) return MI_ERR_WORKSPACE;
+    return MI_OK;
+}
+
+int mi_bin(const mi_particle* d_particles, int64_t n, int64_t n_pairs, const mi_cube_desc* desc,
+           void* d_workspace, size_t workspace_bytes, void* stream) {
+    Grid g;
+    Layout L;
+    int rc = prepare(d_particles, n, n_pairs, desc, d_workspace, workspace_bytes, &g, &L);
+    if (rc) return rc;
+    if (bin_particles(d_particles, n, n_pairs, g, (char*)d_workspace, L, (hipStream_t)stream) !=
+        hipSuccess)
+        return MI_ERR_DEVICE;
+    return MI_OK;
+}
+
+int mi_gather(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
               const mi_cube_desc* desc, float* d_cube, void* d_workspace, size_t workspace_bytes,
               int method, void* stream) {
-    if (!desc_ok(desc) || n < 0 || n_pairs < 0 || !d_cube || !d_workspace) return MI_ERR_ARGUMENT;
-    if (n > 0 && !d_particles) return MI_ERR_ARGUMENT;
+    if (!d_cube) return MI_ERR_ARGUMENT;
     if (method != MI_METHOD_PLAIN_F32 && method != MI_METHOD_MFMA_BF16X3) return MI_ERR_UNSUPPORTED;
-    Grid g = make_grid(*desc);
-    int rc = check_limits(g, n, n_pairs);
-    if (rc) return rc;
+    Grid g;
     Layout L;
-    if (layout_for(g, n, n_pairs, &L) != hipSuccess) return MI_ERR_DEVICE;
-    if (workspace_bytes < L.total) return MI_ERR_WORKSPACE;
+    int rc = prepare(d_particles, n, n_pairs, desc, d_workspace, workspace_bytes, &g, &L);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     char* ws = (char*)d_workspace;
-    if (bin_particles(d_particles, n, n_pairs, g, ws, L, s) != hipSuccess) return MI_ERR_DEVICE;
     hipError_t e = method == MI_METHOD_PLAIN_F32 ? gather_plain(d_particles, g, *desc, ws, L, d_cube, s)
                                                  : gather_mfma(d_particles, g, *desc, ws, L, d_cube, s);
     if (e == hipErrorNotSupported) return MI_ERR_UNSUPPORTED;
     if (e != hipSuccess) return MI_ERR_DEVICE;
     if (reduce_partials(g, ws, L, d_cube, s) != hipSuccess) return MI_ERR_DEVICE;
+    return MI_OK;
+}
+
+int mi_status(const mi_cube_desc* desc, int64_t n, int64_t n_pairs, const void* d_workspace,
+              uint32_t out[4], void* stream) {
+    if (!out) return MI_ERR_ARGUMENT;
+    Grid g;
+    Layout L;
+    if (!desc_ok(desc) || !d_workspace || n < 0 || n_pairs < 0) return MI_ERR_ARGUMENT;
+    g = make_grid(*desc);
+    if (layout_for(g, n, n_pairs, &L) != hipSuccess) return MI_ERR_DEVICE;
     uint32_t st[8];
-    if (hipMemcpyAsync(st, ws + L.status, sizeof(st), hipMemcpyDeviceToHost, s) != hipSuccess)
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(st, (const char*)d_workspace + L.status, sizeof(st), hipMemcpyDeviceToHost,
+                       s) != hipSuccess)
         return MI_ERR_DEVICE;
     if (hipStreamSynchronize(s) != hipSuccess) return MI_ERR_DEVICE;
-    if (st[ST_OVERFLOW]) return MI_ERR_WORKSPACE;
-    return MI_OK;
+    out[0] = st[ST_PAIRS];
+    out[1] = st[ST_OVERFLOW];
+    out[2] = st[ST_ITEMS];
+    out[3] = st[ST_MULTI];
+    return st[ST_OVERFLOW] ? MI_ERR_WORKSPACE : MI_OK;
+}
+
+int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+              const mi_cube_desc* desc, float* d_cube, void* d_workspace, size_t workspace_bytes,
+              int method, void* stream) {
+    if (!d_cube) return MI_ERR_ARGUMENT;
+    if (method != MI_METHOD_PLAIN_F32 && method != MI_METHOD_MFMA_BF16X3) return MI_ERR_UNSUPPORTED;
+    int rc = mi_bin(d_particles, n, n_pairs, desc, d_workspace, workspace_bytes, stream);
+    if (rc) return rc;
+    rc = mi_gather(d_particles, n, n_pairs, desc, d_cube, d_workspace, workspace_bytes, method,
+                   stream);
+    if (rc) return rc;
+    uint32_t st[4];
+    return mi_status(desc, n, n_pairs, d_workspace, st, stream);
 }
 
 int mi_kernel_weights(int kind, float gauss_t, float gauss_inv_norm, const float* d_dx,

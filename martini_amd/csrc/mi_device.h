@@ -6,8 +6,9 @@
 // :994-1044, Dirac :1136-1165).  The polynomial kernels are evaluated in FP64:
 // their closed forms sum terms ~30x larger than the result, which costs FP32
 // about 1e-6 of the particle's peak weight.  The Gaussian kernel and the
-// spectra use FP32 erf/erfc with the difference of two erf values taken on
-// the side where both are small, so tails keep their relative precision.
+// spectra use FP32 differences of erf over one pixel or channel, taken as a
+// midpoint series for narrow spans and on the small-erfc side otherwise, so
+// neither cancels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -150,30 +151,55 @@ __device__ inline float erf_diff(float a, float b) {
     return erff(b) - erff(a);
 }
 
-constexpr float GAUSS_SIG = 0.42466090014400953f;  // 1 / (2 sqrt(2 ln 2)): FWHM = 1
+// erf(m + D/2) - erf(m - D/2) for D >= 0.  Narrow spans use the midpoint
+// Taylor series of the integral of exp(-t^2) (terms to D^7; error below 1e-7
+// of the span's peak for D <= 0.5), which has no cancellation and costs one
+// exp; wider spans take erf_diff.
+__device__ inline float erf_span(float m, float D) {
+    if (D <= 0.5f) {
+        const float m2 = m * m, D2 = D * D;
+        float p = 64.0f * m2 - 480.0f;
+        p = p * m2 + 720.0f;
+        p = p * m2 - 120.0f;                                   // H6-type polynomial
+        float q = (16.0f * m2 - 48.0f) * m2 + 12.0f;            // H4-type polynomial
+        float poly = 1.0f + D2 * ((4.0f * m2 - 2.0f) * (1.0f / 24.0f) +
+                                  D2 * (q * (1.0f / 1920.0f) + D2 * p * (1.0f / 322560.0f)));
+        return 1.12837916709551257f * expf(-m2) * D * poly;
+    }
+    return erf_diff(m - 0.5f * D, m + 0.5f * D);
+}
 
-__device__ inline float w_gauss(float dx, float dy, float h, float t, float inv_norm) {
-    float dr = sqrtf(dx * dx + dy * dy);
-    float hs = h * GAUSS_SIG;
-    float u = dr / hs;
-    if ((dr - 0.70710678118654752f) / hs > t) return 0.0f;  // sph_kernels.py:1041
-    float zmax = (t > u) ? sqrtf(t * t - u * u) : 0.0f;
-    float c = 1.0f / (hs * 1.41421356237309505f);
-    float ex = erf_diff((dx - 0.5f) * c, (dx + 0.5f) * c);
-    float ey = erf_diff((dy - 0.5f) * c, (dy + 0.5f) * c);
+constexpr double GAUSS_SIG = 0.42466090014400953;  // 1 / (2 sqrt(2 ln 2)): FWHM = 1
+
+// The offset and the truncation term zmax = sqrt(t^2 - u^2), u = dr/(h sig),
+// are formed in FP64: zmax has a square-root singularity at the truncation
+// radius, where the t = 3 kernel is still 1% of its peak, so FP32 rounding of
+// u would show there at the 1e-5 level.
+__device__ inline float w_gauss(double dx, double dy, float h, float t, float inv_norm) {
+    const double hs = (double)h * GAUSS_SIG;
+    const double u2 = (dx * dx + dy * dy) / (hs * hs);
+    const double dr = sqrt(dx * dx + dy * dy);
+    if ((dr - 0.70710678118654752) / hs > (double)t) return 0.0f;  // sph_kernels.py:1041
+    const double z2 = (double)t * (double)t - u2;
+    const float zmax = z2 > 0.0 ? (float)sqrt(z2) : 0.0f;
+    const float c = (float)(1.0 / (hs * 1.41421356237309505));  // one pixel in erf-argument units
+    const float ex = erf_span((float)dx * c, c);
+    const float ey = erf_span((float)dy * c, c);
     return 0.25f * erff(zmax * 0.70710678118654752f) * ex * ey * inv_norm;
 }
 
 // Weight of particle p at pixel (i, j): integral of its kernel over the pixel
 // per unit pixel area (sph_kernels.py:85-119 with h = h_px * rescale).
-__device__ inline float pixel_weight(int kind, float dx, float dy, float h, float gt, float ginv) {
+// (dx, dy) = particle - pixel centre, exact in FP64 from (ix - i) + fx.
+__device__ inline float pixel_weight(int kind, double dx, double dy, float h, float gt,
+                                     float ginv) {
     switch (kind) {
-    case MI_KERNEL_CUBIC: return (float)w_cubic((double)dx, (double)dy, (double)h);
-    case MI_KERNEL_WC2: return (float)w_wc2((double)dx, (double)dy, (double)h);
-    case MI_KERNEL_WC6: return (float)w_wc6((double)dx, (double)dy, (double)h);
-    case MI_KERNEL_QUARTIC: return (float)w_quartic((double)dx, (double)dy, (double)h);
+    case MI_KERNEL_CUBIC: return (float)w_cubic(dx, dy, (double)h);
+    case MI_KERNEL_WC2: return (float)w_wc2(dx, dy, (double)h);
+    case MI_KERNEL_WC6: return (float)w_wc6(dx, dy, (double)h);
+    case MI_KERNEL_QUARTIC: return (float)w_quartic(dx, dy, (double)h);
     case MI_KERNEL_GAUSS: return w_gauss(dx, dy, h, gt, ginv);
-    case MI_KERNEL_DIRAC: return (fabsf(dx) < 0.5f && fabsf(dy) < 0.5f) ? 1.0f : 0.0f;
+    case MI_KERNEL_DIRAC: return (fabs(dx) < 0.5 && fabs(dy) < 0.5) ? 1.0f : 0.0f;
     }
     return 0.0f;
 }
@@ -184,8 +210,8 @@ __device__ inline float pixel_weight(int kind, float dx, float dy, float h, floa
 __device__ inline float channel_flux(const mi_particle& p, int c, int spectrum) {
     if (c < p.c_lo || c > p.c_hi) return 0.0f;
     if (spectrum == MI_SPECTRUM_DIRAC) return p.amp;
-    float d = (float)(p.k0 - c) + p.fu;  // u - c
-    return 0.5f * p.amp * erf_diff(p.s * (d - 1.0f), p.s * d);
+    float d = (float)(p.k0 - c) + p.fu;  // u - c: the channel spans s*(d-1) .. s*d
+    return 0.5f * p.amp * erf_span(p.s * (d - 0.5f), p.s);
 }
 
 }  // namespace mi

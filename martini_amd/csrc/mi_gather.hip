@@ -19,80 +19,9 @@
 
 #include "mi_device.h"
 #include "mi_internal.h"
+#include "mi_tile.h"
 
 namespace mi {
-
-constexpr int KC = 32;    // particles per chunk
-constexpr int NT = 256;   // threads per work group
-
-struct ChunkState {
-    mi_particle p[KC];
-    int ilo[KC], ihi[KC], jlo[KC], jhi[KC];
-    int off[KC + 1];
-};
-
-// Loads the chunk's records, clips footprints to the tile and scans the pair
-// counts.  Returns the number of (particle, pixel) pairs via cs.off[nk].
-__device__ inline void load_chunk(const mi_particle* __restrict__ P,
-                                  const uint32_t* __restrict__ order, int kb, int nk, int x0,
-                                  int y0, const Grid& g, ChunkState& cs) {
-    int t = threadIdx.x;
-    if (t < KC) {
-        int cnt = 0;
-        int ilo = 1, ihi = 0, jlo = 1, jhi = 0;
-        if (t < nk) {
-            mi_particle p = P[order[kb + t]];
-            cs.p[t] = p;
-            if (footprint(p, g.nx, g.ny, ilo, ihi, jlo, jhi)) {
-                ilo = max(ilo, x0);
-                ihi = min(ihi, x0 + TILE - 1);
-                jlo = max(jlo, y0);
-                jhi = min(jhi, y0 + TILE - 1);
-                if (ilo <= ihi && jlo <= jhi) cnt = (ihi - ilo + 1) * (jhi - jlo + 1);
-                else { ilo = 1; ihi = 0; jlo = 1; jhi = 0; }
-            } else {
-                ilo = 1; ihi = 0; jlo = 1; jhi = 0;
-            }
-        }
-        cs.ilo[t] = ilo;
-        cs.ihi[t] = ihi;
-        cs.jlo[t] = jlo;
-        cs.jhi[t] = jhi;
-        cs.off[t + 1] = cnt;
-    }
-    __syncthreads();
-    if (t == 0) {
-        cs.off[0] = 0;
-        for (int k = 0; k < KC; ++k) cs.off[k + 1] += cs.off[k];
-    }
-    __syncthreads();
-}
-
-// Evaluates W for every (particle, pixel) pair of the chunk into Wt[k][px].
-__device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
-                                    const mi_cube_desc& d, float (*Wt)[TILE_PX]) {
-    int total = cs.off[KC];
-    for (int q = threadIdx.x; q < total; q += NT) {
-        int lo = 0, hi = KC - 1;  // find k with off[k] <= q < off[k+1]
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (cs.off[mid] <= q) lo = mid;
-            else hi = mid - 1;
-        }
-        int k = lo;
-        int width = cs.jhi[k] - cs.jlo[k] + 1;
-        int local = q - cs.off[k];
-        int a = local / width;
-        int b = local - a * width;
-        int i = cs.ilo[k] + a, j = cs.jlo[k] + b;
-        const mi_particle& p = cs.p[k];
-        float dx = (float)(p.ix - i) + p.fx;
-        float dy = (float)(p.iy - j) + p.fy;
-        int gi = gidx_of(p.r_kind);
-        Wt[k][(i - x0) * TILE + (j - y0)] =
-            pixel_weight(kind_of(p.r_kind), dx, dy, p.h, d.gauss_t[gi], d.gauss_inv_norm[gi]);
-    }
-}
 
 __global__ __launch_bounds__(NT) void k_gather_plain(
     const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
@@ -125,12 +54,12 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
 
     for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
-        load_chunk(P, order, kb, nk, x0, y0, g, cs);
+        load_chunk(P, order, kb, nk, x0, y0, c0, g, cs);
         // zero the weight tile rows of this chunk
         for (int e = t; e < nk * TILE_PX / 4; e += NT)
             reinterpret_cast<float4*>(&Wt[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-        fill_weights(cs, x0, y0, d, Wt);
+        fill_weights(cs, x0, y0, d, [&](int k, int px, float w) { Wt[k][px] = w; });
         for (int e = t; e < nk * CBLOCK; e += NT) {
             int k = e / CBLOCK, cl = e - k * CBLOCK;
             int c = c0 + cl;
@@ -209,7 +138,7 @@ __global__ void k_weights(int kind, float gt, float ginv, const float* __restric
                           const float* __restrict__ dy, const float* __restrict__ h,
                           float* __restrict__ out, int64_t n) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) out[k] = pixel_weight(kind, dx[k], dy[k], h[k], gt, ginv);
+    if (k < n) out[k] = pixel_weight(kind, (double)dx[k], (double)dy[k], h[k], gt, ginv);
 }
 
 __global__ void k_spectra(const mi_particle* __restrict__ P, int64_t n, int nc, int spectrum,

@@ -13,7 +13,7 @@ from tests.helpers import FLUX_TOL, VOXEL_TOL, gpu_cube, oracle_cube, parity
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("method", ["plain"])
+@pytest.mark.parametrize("method", ["plain", "mfma"])
 def test_disc1e4_64px(method):
     w = synthetic.WORKLOADS["disc1e4_64px"]
     ref, P = oracle_cube(w)
@@ -24,7 +24,7 @@ def test_disc1e4_64px(method):
     assert flux <= FLUX_TOL
 
 
-@pytest.mark.parametrize("method", ["plain"])
+@pytest.mark.parametrize("method", ["plain", "mfma"])
 def test_disc1e6_256px(method):
     w = synthetic.WORKLOADS["disc1e6_256px"]
     ref, P = oracle_cube(w)
@@ -35,8 +35,9 @@ def test_disc1e6_256px(method):
     assert flux <= FLUX_TOL
 
 
-def test_deterministic():
+@pytest.mark.parametrize("method", ["plain", "mfma"])
+def test_deterministic(method):
     w = synthetic.WORKLOADS["disc1e4_64px"]
-    a, _ = gpu_cube(w)
-    b, _ = gpu_cube(w)
+    a, _ = gpu_cube(w, method=method)
+    b, _ = gpu_cube(w, method=method)
     assert np.array_equal(a, b)
