@@ -56,6 +56,13 @@ def lib() -> ctypes.CDLL:
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m martini_amd.build_native` "
             "(the insertion path has no CPU fallback)")
+    # torch carries its own HIP runtime; loading torch first makes the library's
+    # libamdhip64 dependency resolve to that same runtime, so device memory and
+    # streams from torch are valid in the library's calls.
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is the device-memory provider
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, f32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
     L.mi_abi_version.restype = ctypes.c_int

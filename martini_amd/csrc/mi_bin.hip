@@ -5,7 +5,7 @@
 // its footprint and channel window touch, and the list is sorted by
 // (cell, kernel group, pixel Morton code) with a stable radix sort, so that
 // each cell's particles form one contiguous, spatially ordered, deterministic
-// segment.  Long segments are cut into work items of at most ITEM_MAX
+// segment.  Long segments are cut into work items of at most item_length()
 // particles; such cells get one partial tile per item.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -42,14 +42,16 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     };
     size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)(n_pairs > 0 ? n_pairs : 1);
     size_t nc1 = (size_t)g.ncells + 1;
-    L->items_max = g.ncells + n_pairs / ITEM_MAX + 1;
-    L->multi_max = 2 * (n_pairs / ITEM_MAX) + 1;
+    L->item_len = item_length(n_pairs);
+    L->items_max = g.ncells + n_pairs / L->item_len + 1;
+    L->multi_max = 2 * (n_pairs / L->item_len) + 1;
     L->counts = take(4 * nn);
     L->offsets = take(4 * nn);
     L->keys = take(4 * np);
     L->vals = take(4 * np);
     L->keys2 = take(4 * np);
     L->vals2 = take(4 * np);
+    L->sorted = take(sizeof(mi_particle) * np);
     L->cell_start = take(4 * (size_t)g.ncells);
     L->cell_end = take(4 * (size_t)g.ncells);
     L->item_counts = take(4 * nc1);
@@ -59,6 +61,7 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     L->items = take(sizeof(Item) * (size_t)L->items_max);
     L->partials = take(sizeof(float) * (size_t)L->multi_max * TILE_PX * CBLOCK);
     L->status = take(4 * 8);
+    L->wtab = take(sizeof(float4) * WTAB_KERNELS * WTAB_N);
     // rocPRIM temporary storage: the largest of the three uses.
     size_t t1 = 0, t2 = 0, t3 = 0;
     uint32_t* nul = nullptr;
@@ -140,6 +143,21 @@ __global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap
     }
 }
 
+// Copies the records into cell order so each work item reads a contiguous
+// stretch of them.  After an overflow the pair buffers hold no valid indices
+// and nothing is read.
+__global__ void k_permute(const mi_particle* __restrict__ P, int64_t n,
+                          const uint32_t* __restrict__ order, int64_t np,
+                          const uint32_t* __restrict__ status, mi_particle* __restrict__ sorted) {
+    if (status[ST_OVERFLOW]) return;
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const uint32_t k = order[q];
+    if ((int64_t)k < n) sorted[q] = P[k];
+}
+
+__global__ void k_fill_wtab(float4* __restrict__ tab) { fill_wtab(tab); }
+
 __global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) a[k] = v;
@@ -158,21 +176,22 @@ __global__ void k_cell_bounds(const uint32_t* __restrict__ keys, int64_t np, int
 }
 
 __global__ void k_item_counts(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                              int64_t ncells, uint32_t* __restrict__ item_counts,
+                              int64_t ncells, int item_len, const uint32_t* __restrict__ status,
+                              uint32_t* __restrict__ item_counts,
                               uint32_t* __restrict__ multi_counts) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c > ncells) return;
     uint32_t ni = 0;
-    if (c < ncells) {
+    if (c < ncells && !status[ST_OVERFLOW] && end[c] > start[c]) {
         uint32_t len = end[c] - start[c];
-        ni = (len + ITEM_MAX - 1) / ITEM_MAX;
+        ni = (len + item_len - 1) / item_len;
     }
     item_counts[c] = ni;
     multi_counts[c] = ni > 1 ? ni : 0;
 }
 
 __global__ void k_build_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                              int64_t ncells, const uint32_t* __restrict__ item_off,
+                              int64_t ncells, int item_len, const uint32_t* __restrict__ item_off,
                               const uint32_t* __restrict__ multi_off, Item* __restrict__ items,
                               uint32_t* __restrict__ status) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -182,12 +201,13 @@ __global__ void k_build_items(const uint32_t* __restrict__ start, const uint32_t
     }
     if (c >= ncells) return;
     uint32_t s0 = start[c], e0 = end[c];
-    uint32_t ni = (e0 - s0 + ITEM_MAX - 1) / ITEM_MAX;
+    const uint32_t L = (uint32_t)item_len;
+    uint32_t ni = (status[ST_OVERFLOW] || e0 <= s0) ? 0u : (e0 - s0 + L - 1) / L;
     for (uint32_t q = 0; q < ni; ++q) {
         Item it;
         it.cell = (int32_t)c;
-        it.begin = (int32_t)(s0 + q * ITEM_MAX);
-        it.end = (int32_t)min(e0, s0 + (q + 1) * ITEM_MAX);
+        it.begin = (int32_t)(s0 + q * L);
+        it.end = (int32_t)min(e0, s0 + (q + 1) * L);
         it.slot = ni > 1 ? (int32_t)(multi_off[c] + q) : -1;
         items[item_off[c] + q] = it;
     }
@@ -216,6 +236,8 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
     const int T = 256;
 
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(T), 0, s, status, (int64_t)8, 0u);
+    hipLaunchKernelGGL(k_fill_wtab, dim3((WTAB_KERNELS * WTAB_N + T - 1) / T), dim3(T), 0, s,
+                       (float4*)(ws + L.wtab));
     if (n > 0) {
         hipLaunchKernelGGL(k_count, dim3(blocks_for(n, T)), dim3(T), 0, s, d_p, n, g, counts);
         hipError_t e = rocprim::exclusive_scan(temp, tb, counts, offsets, 0u, (size_t)n,
@@ -234,13 +256,16 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
                                                  g.cell_bits + KEY_LOCAL_BITS, s);
         if (e != hipSuccess) return e;
     }
+    if (n > 0 && n_pairs > 0)
+        hipLaunchKernelGGL(k_permute, dim3(blocks_for(n_pairs, T)), dim3(T), 0, s, d_p, n, vals2,
+                           n_pairs, status, (mi_particle*)(ws + L.sorted));
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cstart, g.ncells, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cend, g.ncells, 0u);
     if (n_pairs > 0)
         hipLaunchKernelGGL(k_cell_bounds, dim3(blocks_for(n_pairs, T)), dim3(T), 0, s, keys2,
                            n_pairs, g.ncells, cstart, cend);
     hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(g.ncells + 1, T)), dim3(T), 0, s, cstart,
-                       cend, g.ncells, icount, mcount);
+                       cend, g.ncells, L.item_len, status, icount, mcount);
     hipError_t e = rocprim::exclusive_scan(temp, tb, icount, ioff, 0u, (size_t)g.ncells + 1,
                                            rocprim::plus<uint32_t>(), s);
     if (e != hipSuccess) return e;
@@ -248,7 +273,7 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
                                 rocprim::plus<uint32_t>(), s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_build_items, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cstart, cend,
-                       g.ncells, ioff, moff, items, status);
+                       g.ncells, L.item_len, ioff, moff, items, status);
     return hipGetLastError();
 }
 

@@ -3,9 +3,9 @@
 //
 // Kernel integrals restate martini/sph_kernels.py (cubic :791-858,
 // WendlandC2 :400-441, WendlandC6 :544-685, quartic :1486-1564, Gaussian
-// :994-1044, Dirac :1136-1165).  The polynomial kernels are evaluated in FP64:
-// their closed forms sum terms ~30x larger than the result, which costs FP32
-// about 1e-6 of the particle's peak weight.  The Gaussian kernel and the
+// :994-1044, Dirac :1136-1165).  The closed forms sum terms ~30x larger than
+// their result, so they are evaluated once in FP64 into an interpolation table
+// over q = dr/h, which the insertion kernels read.  The Gaussian kernel and the
 // spectra use FP32 differences of erf over one pixel or channel, taken as a
 // midpoint series for narrow spans and on the small-erfc side otherwise, so
 // neither cancels.
@@ -22,9 +22,19 @@ namespace mi {
 constexpr int TILE = 16;
 constexpr int TILE_PX = TILE * TILE;
 constexpr int CBLOCK = 64;
-// Particles per cell processed by one work item (longer cells are split and
-// their partial tiles summed in item order by a reduction pass).
+// Particles per cell processed by one work item: longer cells are split and
+// their partial tiles summed in item order by a reduction pass.  The split
+// length depends only on the pair count (about ITEM_TARGET items, between
+// ITEM_MIN and ITEM_MAX particles each), so a given input always sums in the
+// same order.
+constexpr int ITEM_MIN = 256;
 constexpr int ITEM_MAX = 4096;
+constexpr int ITEM_TARGET = 2048;
+
+__host__ __device__ inline int item_length(int64_t n_pairs) {
+    int64_t len = (n_pairs / ITEM_TARGET + 31) / 32 * 32;
+    return (int)(len < ITEM_MIN ? ITEM_MIN : (len > ITEM_MAX ? ITEM_MAX : len));
+}
 // Key layout: cell << KEY_LOCAL_BITS | kernel group << 8 | 8-bit pixel Morton code.
 constexpr int KEY_LOCAL_BITS = 10;
 
@@ -52,7 +62,7 @@ __host__ __device__ inline bool footprint(const mi_particle& p, int nx, int ny, 
 }
 
 // ---------------------------------------------------------------- kernels
-__device__ inline double w_cubic(double dx, double dy, double h) {
+__host__ __device__ inline double w_cubic(double dx, double dy, double h) {
     double R2 = 4.0 * (dx * dx + dy * dy) / (h * h), F;
     if (R2 == 0.0) {
         F = 0.75;
@@ -77,7 +87,7 @@ __device__ inline double w_cubic(double dx, double dy, double h) {
     return F * (4.0 / 1.59689476201133) / (h * h);
 }
 
-__device__ inline double w_wc2(double dx, double dy, double h) {
+__host__ __device__ inline double w_wc2(double dx, double dy, double h) {
     double R2 = (dx * dx + dy * dy) / (h * h), G;
     if (R2 == 0.0) {
         G = 2.0 / 3.0;
@@ -93,7 +103,7 @@ __device__ inline double w_wc2(double dx, double dy, double h) {
 
 // WendlandC6 indefinite integral in z at fixed R (the reference's rounded
 // coefficients), grouped by the factor each term carries.
-__device__ inline double wc6_indef(double R, double z) {
+__host__ __device__ inline double wc6_indef(double R, double z) {
     double R2 = R * R, z2 = z * z;
     double S = sqrt(R2 + z2), L = log(S + z);
     double R4 = R2 * R2, R6 = R4 * R2, R8 = R4 * R4, R10 = R8 * R2, R12 = R6 * R6;
@@ -112,7 +122,7 @@ __device__ inline double wc6_indef(double R, double z) {
     return plain + withS * S + withL * L;
 }
 
-__device__ inline double w_wc6(double dx, double dy, double h) {
+__host__ __device__ inline double w_wc6(double dx, double dy, double h) {
     double R = sqrt(dx * dx + dy * dy) / h, v;
     const double norm = 1365.0 / (64.0 * M_PI);
     if (R == 0.0) v = norm * 2.0 * (4.0 / 15.0);
@@ -122,7 +132,7 @@ __device__ inline double w_wc6(double dx, double dy, double h) {
 }
 
 // Indefinite integral over z of (A - sqrt(R^2+z^2))^4 at fixed R.
-__device__ inline double quartic_IA(double R, double z, double A) {
+__host__ __device__ inline double quartic_IA(double R, double z, double A) {
     double q = sqrt(z * z + R * R), R2 = R * R, A2 = A * A;
     return A2 * A2 * z - 2.0 * A2 * A * z * q + 2.0 * A2 * z * (3.0 * R2 + z * z) -
            0.5 * A * R2 * (4.0 * A2 + 3.0 * R2) * asinh(z / R) -
@@ -130,7 +140,7 @@ __device__ inline double quartic_IA(double R, double z, double A) {
            0.2 * z * z * z * z * z;
 }
 
-__device__ inline double w_quartic(double dx, double dy, double h) {
+__host__ __device__ inline double w_quartic(double dx, double dy, double h) {
     double R = sqrt(dx * dx + dy * dy) / h, v = 0.0;
     if (R == 0.0) {
         v = 384.0 / 3125.0;
@@ -177,9 +187,10 @@ constexpr double GAUSS_SIG = 0.42466090014400953;  // 1 / (2 sqrt(2 ln 2)): FWHM
 // u would show there at the 1e-5 level.
 __device__ inline float w_gauss(double dx, double dy, float h, float t, float inv_norm) {
     const double hs = (double)h * GAUSS_SIG;
-    const double u2 = (dx * dx + dy * dy) / (hs * hs);
-    const double dr = sqrt(dx * dx + dy * dy);
-    if ((dr - 0.70710678118654752) / hs > (double)t) return 0.0f;  // sph_kernels.py:1041
+    const double dr2 = dx * dx + dy * dy;
+    const double u2 = dr2 / (hs * hs);
+    const double edge = (double)t * hs + 0.70710678118654752;  // sph_kernels.py:1041
+    if (dr2 > edge * edge) return 0.0f;
     const double z2 = (double)t * (double)t - u2;
     const float zmax = z2 > 0.0 ? (float)sqrt(z2) : 0.0f;
     const float c = (float)(1.0 / (hs * 1.41421356237309505));  // one pixel in erf-argument units
@@ -188,20 +199,90 @@ __device__ inline float w_gauss(double dx, double dy, float h, float t, float in
     return 0.25f * erff(zmax * 0.70710678118654752f) * ex * ey * inv_norm;
 }
 
+// The four closed-form kernels depend on q = dr/h only: w = F(q) / h^2 with
+// F(q) = w(q, 0, 1).  F is tabulated at WTAB_N intervals of [0, 1] from the
+// FP64 closed forms above; entry i holds F at nodes i-1 .. i+2 (F(-q) = F(q))
+// and the weight is the 4-point Lagrange interpolant.  WTAB_N = 500 puts the
+// kernels' seams (cubic q = 1/2, quartic q = 1/5 and 3/5) on nodes; the
+// error, about dq^4 max|F^(4)| / 24, stays below 1e-9 of F(0).  Node WTAB_N
+// holds the left limit F(1-): WendlandC6's rounded coefficients leave it
+// nonzero (~4e-5 of F(0)), so its node beyond 1 is extrapolated from inside;
+// the others vanish smoothly at q = 1.  q >= 1 gives 0, as the reference's
+// strict or zero-valued support edge does.
+constexpr int WTAB_N = 500;
+constexpr int WTAB_KERNELS = 4;  // indexed by kernel kind: cubic, WC2, WC6, quartic
+
+// The reference's cubic I3 exceeds the exact outer-shell integral by R2*A/8
+// (R2 = 4 q^2 <= 1, A = sqrt(1 - R2)); with its 1/4 and normalisation this adds
+// R2*A/(8 * 1.59689476201133) to the weight.  Its square-root behaviour at
+// the seam q = 1/2 is kept out of the table and added analytically.
+__host__ __device__ inline double cubic_seam_term(double q) {
+    const double R2 = 4.0 * q * q;
+    return R2 <= 1.0 ? R2 * sqrt(1.0 - R2) / (8.0 * 1.59689476201133) : 0.0;
+}
+
+__host__ __device__ inline double wtab_eval(int which, double q) {
+    switch (which) {
+    case MI_KERNEL_CUBIC: return w_cubic(q, 0.0, 1.0) - cubic_seam_term(q);
+    case MI_KERNEL_WC2: return w_wc2(q, 0.0, 1.0);
+    case MI_KERNEL_WC6: return w_wc6(q, 0.0, 1.0);
+    default: return w_quartic(q, 0.0, 1.0);
+    }
+}
+
+__host__ __device__ inline double wtab_node(int which, int node) {
+    const int a = node < 0 ? -node : node;
+    if (a < WTAB_N) return wtab_eval(which, (double)a / WTAB_N);
+    const double left = wtab_eval(which, 1.0 - 1e-13);  // F(1-)
+    if (a == WTAB_N) return left;
+    if (which != MI_KERNEL_WC6) return 0.0;
+    return 3.0 * left - 3.0 * wtab_eval(which, (double)(WTAB_N - 1) / WTAB_N) +
+           wtab_eval(which, (double)(WTAB_N - 2) / WTAB_N);
+}
+
+__device__ inline float table_weight(const float4* __restrict__ tab, float q) {
+    if (!(q < 1.0f)) return 0.0f;
+    const float x = q * (float)WTAB_N;
+    const int i = min((int)x, WTAB_N - 1);
+    const float t = x - (float)i;
+    const float4 f = tab[i];
+    const float tp1 = t + 1.0f, tm1 = t - 1.0f, tm2 = t - 2.0f;
+    const float l0 = -t * tm1 * tm2 * (1.0f / 6.0f);
+    const float l1 = tp1 * tm1 * tm2 * 0.5f;
+    const float l2 = -tp1 * t * tm2 * 0.5f;
+    const float l3 = tp1 * t * tm1 * (1.0f / 6.0f);
+    return l0 * f.x + l1 * f.y + l2 * f.z + l3 * f.w;
+}
+
 // Weight of particle p at pixel (i, j): integral of its kernel over the pixel
 // per unit pixel area (sph_kernels.py:85-119 with h = h_px * rescale).
 // (dx, dy) = particle - pixel centre, exact in FP64 from (ix - i) + fx.
+// wtab: WTAB_KERNELS x WTAB_N entries filled by fill_wtab.
 __device__ inline float pixel_weight(int kind, double dx, double dy, float h, float gt,
-                                     float ginv) {
-    switch (kind) {
-    case MI_KERNEL_CUBIC: return (float)w_cubic(dx, dy, (double)h);
-    case MI_KERNEL_WC2: return (float)w_wc2(dx, dy, (double)h);
-    case MI_KERNEL_WC6: return (float)w_wc6(dx, dy, (double)h);
-    case MI_KERNEL_QUARTIC: return (float)w_quartic(dx, dy, (double)h);
-    case MI_KERNEL_GAUSS: return w_gauss(dx, dy, h, gt, ginv);
-    case MI_KERNEL_DIRAC: return (fabs(dx) < 0.5 && fabs(dy) < 0.5) ? 1.0f : 0.0f;
+                                     float ginv, const float4* __restrict__ wtab) {
+    if (kind <= MI_KERNEL_QUARTIC) {
+        const double hd = (double)h;
+        const float q2 = (float)((dx * dx + dy * dy) / (hd * hd));
+        const float q = sqrtf(q2);
+        float f = table_weight(wtab + kind * WTAB_N, q);
+        if (kind == MI_KERNEL_CUBIC && q2 <= 0.25f) {
+            const float R2 = 4.0f * q2;
+            f += R2 * sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
+        }
+        return f / (h * h);
     }
+    if (kind == MI_KERNEL_GAUSS) return w_gauss(dx, dy, h, gt, ginv);
+    if (kind == MI_KERNEL_DIRAC) return (fabs(dx) < 0.5 && fabs(dy) < 0.5) ? 1.0f : 0.0f;
     return 0.0f;
+}
+
+// Fills the weight table: one thread per entry (WTAB_KERNELS * WTAB_N threads).
+__device__ inline void fill_wtab(float4* __restrict__ tab) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= WTAB_KERNELS * WTAB_N) return;
+    const int which = e / WTAB_N, i = e - which * WTAB_N;
+    tab[e] = make_float4((float)wtab_node(which, i - 1), (float)wtab_node(which, i),
+                         (float)wtab_node(which, i + 1), (float)wtab_node(which, i + 2));
 }
 
 // Spectrum of particle p in channel c (Jy per unit weight): amp times the

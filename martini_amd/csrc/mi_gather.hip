@@ -24,19 +24,17 @@
 namespace mi {
 
 __global__ __launch_bounds__(NT) void k_gather_plain(
-    const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
-    const Item* __restrict__ items, const uint32_t* __restrict__ status, Grid g, mi_cube_desc d,
+    const mi_particle* __restrict__ sorted, const Item* __restrict__ items,
+    const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
     float* __restrict__ cube, float* __restrict__ partials) {
     if (status[ST_OVERFLOW]) return;
     const int item_id = blockIdx.x;
     if ((uint32_t)item_id >= status[ST_ITEMS]) return;
     const Item it = items[item_id];
-    const int cb = it.cell % g.ncb;
-    const int txy = it.cell / g.ncb;
-    const int ty = txy % g.nty, tx = txy / g.nty;
-    const int x0 = tx * TILE, y0 = ty * TILE, c0 = cb * CBLOCK;
+    const CellPos cp = cell_pos(it.cell, g);
+    const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
 
-    __shared__ float Wt[KC][TILE_PX];
+    __shared__ __attribute__((aligned(16))) float Wt[KC][TILE_PX];
     __shared__ __attribute__((aligned(16))) float St[KC][CBLOCK];
     __shared__ ChunkState cs;
 
@@ -52,17 +50,23 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) acc[jj][cc] = 0.0f;
 
+    mi_particle next;  // wave 0: the record this lane stages next
+    if (wave == 0 && t < min(KC, it.end - it.begin)) next = sorted[it.begin + t];
+
     for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
-        load_chunk(P, order, kb, nk, x0, y0, c0, g, cs);
-        // zero the weight tile rows of this chunk
+        if (wave == 0) {
+            stage_chunk(next, t < nk, x0, y0, c0, g, cs);
+            const int nk2 = min(KC, it.end - kb - KC);
+            if (t < nk2) next = sorted[kb + KC + t];
+        }
         for (int e = t; e < nk * TILE_PX / 4; e += NT)
             reinterpret_cast<float4*>(&Wt[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-        fill_weights(cs, x0, y0, d, [&](int k, int px, float w) { Wt[k][px] = w; });
+        fill_weights(cs, x0, y0, d, wtab, [&](int k, int px, float w) { Wt[k][px] = w; });
         for (int e = t; e < nk * CBLOCK; e += NT) {
-            int k = e / CBLOCK, cl = e - k * CBLOCK;
-            int c = c0 + cl;
+            const int k = e / CBLOCK, cl = e - k * CBLOCK;
+            const int c = c0 + cl;
             St[k][cl] = c < g.nc ? channel_flux(cs.p[k], c, d.spectrum) : 0.0f;
         }
         __syncthreads();
@@ -91,12 +95,12 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
         if (i < g.nx) {
 #pragma unroll
             for (int jj = 0; jj < TILE; ++jj) {
-                int j = y0 + jj;
+                const int j = y0 + jj;
                 if (j >= g.ny) continue;
                 float* dst = cube + ((size_t)i * g.ny + j) * g.nc;
 #pragma unroll
                 for (int cc = 0; cc < 4; ++cc) {
-                    int c = c0 + cg * 4 + cc;
+                    const int c = c0 + cg * 4 + cc;
                     if (c < g.nc) dst[c] += acc[jj][cc];
                 }
             }
@@ -111,34 +115,61 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
 }
 
 // Sums the partial tiles of each split cell in item order and adds them to
-// the cube.
+// the cube.  Block (cell, part): 256 threads x 4 voxels cover a 1024-voxel
+// part of the 16384-voxel tile; the item loop is unrolled so its loads are
+// independent.
+constexpr int RED_PARTS = TILE_PX * CBLOCK / (NT * 4);
+
 __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item_counts,
                                                const uint32_t* __restrict__ multi_off,
                                                const uint32_t* __restrict__ status, Grid g,
                                                const float* __restrict__ partials,
                                                float* __restrict__ cube) {
     if (status[ST_OVERFLOW]) return;
-    const int cell = blockIdx.x;
+    const int cell = blockIdx.x / RED_PARTS;
+    const int part = blockIdx.x - cell * RED_PARTS;
     const uint32_t ni = item_counts[cell];
     if (ni <= 1) return;
-    const float* base = partials + (size_t)multi_off[cell] * TILE_PX * CBLOCK;
-    const int cb = cell % g.ncb;
-    const int txy = cell / g.ncb;
-    const int ty = txy % g.nty, tx = txy / g.nty;
-    for (int v = threadIdx.x; v < TILE_PX * CBLOCK; v += NT) {
-        int px = v / CBLOCK, cl = v - px * CBLOCK;
-        int i = tx * TILE + px / TILE, j = ty * TILE + px % TILE, c = cb * CBLOCK + cl;
-        float s = 0.0f;
-        for (uint32_t q = 0; q < ni; ++q) s += base[(size_t)q * TILE_PX * CBLOCK + v];
-        if (i < g.nx && j < g.ny && c < g.nc) cube[((size_t)i * g.ny + j) * g.nc + c] += s;
+    const size_t tile = (size_t)TILE_PX * CBLOCK / 4;  // float4 per partial tile
+    const float4* base = reinterpret_cast<const float4*>(partials) + (size_t)multi_off[cell] * tile;
+    const int v4 = part * NT + threadIdx.x;  // float4 index within the tile
+    float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t q = 0;
+    for (; q + 4 <= ni; q += 4) {
+        const float4 a = base[(q + 0) * tile + v4], b = base[(q + 1) * tile + v4];
+        const float4 c = base[(q + 2) * tile + v4], e = base[(q + 3) * tile + v4];
+        s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
+        s0.x += b.x; s0.y += b.y; s0.z += b.z; s0.w += b.w;
+        s0.x += c.x; s0.y += c.y; s0.z += c.z; s0.w += c.w;
+        s0.x += e.x; s0.y += e.y; s0.z += e.z; s0.w += e.w;
     }
+    for (; q < ni; ++q) {
+        const float4 a = base[q * tile + v4];
+        s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
+    }
+    const CellPos cp = cell_pos(cell, g);
+    const int px = (v4 * 4) / CBLOCK, cl = (v4 * 4) % CBLOCK;
+    const int i = cp.x0 + px / TILE, j = cp.y0 + px % TILE;
+    if (i >= g.nx || j >= g.ny) return;
+    float* dst = cube + ((size_t)i * g.ny + j) * g.nc + cp.c0 + cl;
+    const float sv[4] = {s0.x, s0.y, s0.z, s0.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (cp.c0 + cl + u < g.nc) dst[u] += sv[u];
 }
+
+// Weight table of the plugin-level entry point (mi_kernel_weights), refilled
+// before each of its launches on the same stream.
+__device__ float4 g_api_wtab[WTAB_KERNELS * WTAB_N];
+
+__global__ void k_fill_api_wtab() { fill_wtab(g_api_wtab); }
 
 __global__ void k_weights(int kind, float gt, float ginv, const float* __restrict__ dx,
                           const float* __restrict__ dy, const float* __restrict__ h,
                           float* __restrict__ out, int64_t n) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) out[k] = pixel_weight(kind, (double)dx[k], (double)dy[k], h[k], gt, ginv);
+    if (k < n)
+        out[k] = pixel_weight(kind, (double)dx[k], (double)dy[k], h[k], gt, ginv, g_api_wtab);
 }
 
 __global__ void k_spectra(const mi_particle* __restrict__ P, int64_t n, int nc, int spectrum,
@@ -153,15 +184,16 @@ __global__ void k_spectra(const mi_particle* __restrict__ P, int64_t n, int nc, 
 hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
                         const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_plain, dim3((unsigned)L.items_max), dim3(NT), 0, s, d_p,
-                       (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
-                       (const uint32_t*)(ws + L.status), g, d, d_cube, (float*)(ws + L.partials));
+    hipLaunchKernelGGL(k_gather_plain, dim3((unsigned)L.items_max), dim3(NT), 0, s,
+                       (const mi_particle*)(ws + L.sorted), (const Item*)(ws + L.items),
+                       (const uint32_t*)(ws + L.status), (const float4*)(ws + L.wtab), g, d,
+                       d_cube, (float*)(ws + L.partials));
     return hipGetLastError();
 }
 
 hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cube,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)g.ncells), dim3(NT), 0, s,
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)(g.ncells * RED_PARTS)), dim3(NT), 0, s,
                        (const uint32_t*)(ws + L.item_counts), (const uint32_t*)(ws + L.multi_off),
                        (const uint32_t*)(ws + L.status), g, (const float*)(ws + L.partials),
                        d_cube);
@@ -171,6 +203,7 @@ hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cu
 hipError_t launch_weights(int kind, float gt, float ginv, const float* dx, const float* dy,
                           const float* h, float* out, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_api_wtab, dim3((WTAB_KERNELS * WTAB_N + 255) / 256), dim3(256), 0, s);
     hipLaunchKernelGGL(k_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kind, gt,
                        ginv, dx, dy, h, out, n);
     return hipGetLastError();

@@ -42,16 +42,19 @@ __host__ __device__ inline uint32_t cells_of(const mi_particle& p, const Grid& g
 struct Layout {
     size_t counts, offsets;            // uint32[n] each
     size_t keys, vals, keys2, vals2;   // uint32[n_pairs] each
+    size_t sorted;                     // mi_particle[n_pairs]: records in cell order
     size_t cell_start, cell_end;       // uint32[ncells] each
     size_t item_counts, item_off;      // uint32[ncells + 1] each
     size_t multi_counts, multi_off;    // uint32[ncells + 1] each
     size_t items;                      // int4[items_max]
     size_t partials;                   // float[multi_max * TILE_PX * CBLOCK]
     size_t status;                     // uint32[8]
+    size_t wtab;                       // float4[WTAB_KERNELS * WTAB_N] weight table
     size_t temp;                       // rocPRIM temporary storage
     size_t temp_bytes;
     size_t total;
     int64_t items_max, multi_max;
+    int item_len;                      // particles per work item
 };
 
 // status[0]: number of pairs the device produced; status[1]: overflow flag.

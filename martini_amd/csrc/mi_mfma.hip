@@ -61,17 +61,15 @@ __device__ inline floatx4 mfma3(const shortx8& ah, const shortx8& al, const shor
 }
 
 __global__ __launch_bounds__(NT) void k_gather_mfma(
-    const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
-    const Item* __restrict__ items, const uint32_t* __restrict__ status, Grid g, mi_cube_desc d,
+    const mi_particle* __restrict__ sorted, const Item* __restrict__ items,
+    const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
     float* __restrict__ cube, float* __restrict__ partials) {
     if (status[ST_OVERFLOW]) return;
     const int item_id = blockIdx.x;
     if ((uint32_t)item_id >= status[ST_ITEMS]) return;
     const Item it = items[item_id];
-    const int cb = it.cell % g.ncb;
-    const int txy = it.cell / g.ncb;
-    const int ty = txy % g.nty, tx = txy / g.nty;
-    const int x0 = tx * TILE, y0 = ty * TILE, c0 = cb * CBLOCK;
+    const CellPos cp = cell_pos(it.cell, g);
+    const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
 
     __shared__ __attribute__((aligned(16))) float Wp[TILE_PX][KPAD];  // [pixel][k]
     __shared__ __attribute__((aligned(16))) float Sp[CBLOCK][KPAD];   // [channel][k]
@@ -89,13 +87,20 @@ __global__ __launch_bounds__(NT) void k_gather_mfma(
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) acc[r][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
 
+    mi_particle next;  // wave 0: the record this lane stages next
+    if (wave == 0 && t < min(KC, it.end - it.begin)) next = sorted[it.begin + t];
+
     for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
-        load_chunk(P, order, kb, nk, x0, y0, c0, g, cs);
+        if (wave == 0) {
+            stage_chunk(next, t < nk, x0, y0, c0, g, cs);
+            const int nk2 = min(KC, it.end - kb - KC);
+            if (t < nk2) next = sorted[kb + KC + t];
+        }
         for (int e = t; e < TILE_PX * KPAD / 4; e += NT)
             reinterpret_cast<float4*>(&Wp[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-        fill_weights(cs, x0, y0, d, [&](int k, int px, float w) { Wp[px][k] = w; });
+        fill_weights(cs, x0, y0, d, wtab, [&](int k, int px, float w) { Wp[px][k] = w; });
         for (int e = t; e < CBLOCK * KC; e += NT) {
             int cl = e / KC, k = e - cl * KC;
             int c = c0 + cl;
@@ -146,9 +151,10 @@ __global__ __launch_bounds__(NT) void k_gather_mfma(
 hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
                        const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_mfma, dim3((unsigned)L.items_max), dim3(NT), 0, s, d_p,
-                       (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
-                       (const uint32_t*)(ws + L.status), g, d, d_cube, (float*)(ws + L.partials));
+    hipLaunchKernelGGL(k_gather_mfma, dim3((unsigned)L.items_max), dim3(NT), 0, s,
+                       (const mi_particle*)(ws + L.sorted), (const Item*)(ws + L.items),
+                       (const uint32_t*)(ws + L.status), (const float4*)(ws + L.wtab), g, d,
+                       d_cube, (float*)(ws + L.partials));
     return hipGetLastError();
 }
 
