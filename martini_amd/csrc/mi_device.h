@@ -181,24 +181,6 @@ __device__ inline float erf_span(float m, float D) {
 
 constexpr double GAUSS_SIG = 0.42466090014400953;  // 1 / (2 sqrt(2 ln 2)): FWHM = 1
 
-// The offset and the truncation term zmax = sqrt(t^2 - u^2), u = dr/(h sig),
-// are formed in FP64: zmax has a square-root singularity at the truncation
-// radius, where the t = 3 kernel is still 1% of its peak, so FP32 rounding of
-// u would show there at the 1e-5 level.
-__device__ inline float w_gauss(double dx, double dy, float h, float t, float inv_norm) {
-    const double hs = (double)h * GAUSS_SIG;
-    const double dr2 = dx * dx + dy * dy;
-    const double u2 = dr2 / (hs * hs);
-    const double edge = (double)t * hs + 0.70710678118654752;  // sph_kernels.py:1041
-    if (dr2 > edge * edge) return 0.0f;
-    const double z2 = (double)t * (double)t - u2;
-    const float zmax = z2 > 0.0 ? (float)sqrt(z2) : 0.0f;
-    const float c = (float)(1.0 / (hs * 1.41421356237309505));  // one pixel in erf-argument units
-    const float ex = erf_span((float)dx * c, c);
-    const float ey = erf_span((float)dy * c, c);
-    return 0.25f * erff(zmax * 0.70710678118654752f) * ex * ey * inv_norm;
-}
-
 // The four closed-form kernels depend on q = dr/h only: w = F(q) / h^2 with
 // F(q) = w(q, 0, 1).  F is tabulated at WTAB_N intervals of [0, 1] from the
 // FP64 closed forms above; entry i holds F at nodes i-1 .. i+2 (F(-q) = F(q))
@@ -254,26 +236,70 @@ __device__ inline float table_weight(const float4* __restrict__ tab, float q) {
     return l0 * f.x + l1 * f.y + l2 * f.z + l3 * f.w;
 }
 
-// Weight of particle p at pixel (i, j): integral of its kernel over the pixel
-// per unit pixel area (sph_kernels.py:85-119 with h = h_px * rescale).
-// (dx, dy) = particle - pixel centre, exact in FP64 from (ix - i) + fx.
-// wtab: WTAB_KERNELS x WTAB_N entries filled by fill_wtab.
-__device__ inline float pixel_weight(int kind, double dx, double dy, float h, float gt,
-                                     float ginv, const float4* __restrict__ wtab) {
+// Per-particle factors of the pixel weight, formed once per particle so the
+// per-pixel evaluation has no division:
+//   table kernels: a = 1/h^2 (FP64, for q^2), b = (float)(1/h^2);
+//   Gaussian:      a = 1/(h sig)^2, e = (t h sig + 1/sqrt 2)^2 (the truncation
+//                  edge of sph_kernels.py:1041, squared), b = 1/(h sig sqrt 2)
+//                  (one pixel in erf-argument units), t and n = 1/norm_t.
+struct WFactors {
+    double a, e;
+    float b, t, n;
+};
+
+__device__ inline WFactors weight_factors(int kind, float h, float gt, float ginv) {
+    WFactors f{0.0, 0.0, 0.0f, gt, ginv};
     if (kind <= MI_KERNEL_QUARTIC) {
         const double hd = (double)h;
-        const float q2 = (float)((dx * dx + dy * dy) / (hd * hd));
+        f.a = 1.0 / (hd * hd);
+        f.b = (float)f.a;
+    } else if (kind == MI_KERNEL_GAUSS) {
+        const double hs = (double)h * GAUSS_SIG;
+        f.a = 1.0 / (hs * hs);
+        const double edge = (double)gt * hs + 0.70710678118654752;
+        f.e = edge * edge;
+        f.b = (float)(1.0 / (hs * 1.41421356237309505));
+    }
+    return f;
+}
+
+// Weight of a particle at pixel offset (dx, dy) = particle - pixel centre,
+// exact in FP64 from (ix - i) + fx: the integral of its kernel over the pixel
+// per unit pixel area (sph_kernels.py:85-119 with h = h_px * rescale).
+// wtab: WTAB_KERNELS x WTAB_N entries filled by fill_wtab.
+//
+// Gaussian: the offset and the truncation term zmax = sqrt(t^2 - u^2),
+// u = dr/(h sig), are formed in FP64: zmax has a square-root singularity at
+// the truncation radius, where the t = 3 kernel is still 1% of its peak, so
+// FP32 rounding of u would show there at the 1e-5 level.
+__device__ inline float weight_at(int kind, double dx, double dy, const WFactors& f,
+                                  const float4* __restrict__ wtab) {
+    const double dr2 = dx * dx + dy * dy;
+    if (kind <= MI_KERNEL_QUARTIC) {
+        const float q2 = (float)(dr2 * f.a);
         const float q = sqrtf(q2);
-        float f = table_weight(wtab + kind * WTAB_N, q);
+        float w = table_weight(wtab + kind * WTAB_N, q);
         if (kind == MI_KERNEL_CUBIC && q2 <= 0.25f) {
             const float R2 = 4.0f * q2;
-            f += R2 * sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
+            w += R2 * sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
         }
-        return f / (h * h);
+        return w * f.b;
     }
-    if (kind == MI_KERNEL_GAUSS) return w_gauss(dx, dy, h, gt, ginv);
+    if (kind == MI_KERNEL_GAUSS) {
+        if (dr2 > f.e) return 0.0f;
+        const double z2 = (double)f.t * (double)f.t - dr2 * f.a;
+        const float zmax = z2 > 0.0 ? (float)sqrt(z2) : 0.0f;
+        const float ex = erf_span((float)dx * f.b, f.b);
+        const float ey = erf_span((float)dy * f.b, f.b);
+        return 0.25f * erff(zmax * 0.70710678118654752f) * ex * ey * f.n;
+    }
     if (kind == MI_KERNEL_DIRAC) return (fabs(dx) < 0.5 && fabs(dy) < 0.5) ? 1.0f : 0.0f;
     return 0.0f;
+}
+
+__device__ inline float pixel_weight(int kind, double dx, double dy, float h, float gt,
+                                     float ginv, const float4* __restrict__ wtab) {
+    return weight_at(kind, dx, dy, weight_factors(kind, h, gt, ginv), wtab);
 }
 
 // Fills the weight table: one thread per entry (WTAB_KERNELS * WTAB_N threads).
@@ -285,14 +311,20 @@ __device__ inline void fill_wtab(float4* __restrict__ tab) {
                          (float)wtab_node(which, i + 1), (float)wtab_node(which, i + 2));
 }
 
-// Spectrum of particle p in channel c (Jy per unit weight): amp times the
+// Spectrum of a particle in channel c (Jy per unit weight): amp times the
 // Gaussian line integrated over the channel (spectral_models.py:347-425), or
-// amp inside the Dirac window (:510-570).
+// amp inside the Dirac window (:510-570).  [lo, hi] is the particle's channel
+// window, u = k0 + fu its line centre in channel units, s = |dv|/(sqrt 2 sigma).
+__device__ inline float line_flux(int lo, int hi, int k0, float fu, float s, float amp, int c,
+                                  int spectrum) {
+    if (c < lo || c > hi) return 0.0f;
+    if (spectrum == MI_SPECTRUM_DIRAC) return amp;
+    const float dd = (float)(k0 - c) + fu;  // u - c: the channel spans s*(dd-1) .. s*dd
+    return 0.5f * amp * erf_span(s * (dd - 0.5f), s);
+}
+
 __device__ inline float channel_flux(const mi_particle& p, int c, int spectrum) {
-    if (c < p.c_lo || c > p.c_hi) return 0.0f;
-    if (spectrum == MI_SPECTRUM_DIRAC) return p.amp;
-    float d = (float)(p.k0 - c) + p.fu;  // u - c: the channel spans s*(d-1) .. s*d
-    return 0.5f * p.amp * erf_span(p.s * (d - 0.5f), p.s);
+    return line_flux(p.c_lo, p.c_hi, p.k0, p.fu, p.s, p.amp, c, spectrum);
 }
 
 }  // namespace mi

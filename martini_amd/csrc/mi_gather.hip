@@ -39,10 +39,14 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
     __shared__ ChunkState cs;
 
     const int t = threadIdx.x;
-    const int wave = t >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
     const int pg = t >> 4;  // tile row (pixel offset along x)
     const int cg = t & 15;  // 4-channel group
+    const int lane = t & 63;
     const int wrow0 = x0 + wave * 4, wrow1 = wrow0 + 3;
+
+    for (int e = t; e < KC * TILE_PX / 4; e += NT)
+        reinterpret_cast<float4*>(&Wt[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
 
     float acc[TILE][4];
 #pragma unroll
@@ -56,22 +60,22 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
     for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
         if (wave == 0) {
-            stage_chunk(next, t < nk, x0, y0, c0, g, cs);
+            stage_chunk(next, t < nk, x0, y0, c0, g, d, cs);
             const int nk2 = min(KC, it.end - kb - KC);
             if (t < nk2) next = sorted[kb + KC + t];
         }
-        for (int e = t; e < nk * TILE_PX / 4; e += NT)
-            reinterpret_cast<float4*>(&Wt[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-        fill_weights(cs, x0, y0, d, wtab, [&](int k, int px, float w) { Wt[k][px] = w; });
-        for (int e = t; e < nk * CBLOCK; e += NT) {
-            const int k = e / CBLOCK, cl = e - k * CBLOCK;
-            const int c = c0 + cl;
-            St[k][cl] = c < g.nc ? channel_flux(cs.p[k], c, d.spectrum) : 0.0f;
-        }
+        // Particles whose footprint meets this wave's 4 rows (wave-uniform).
+        const int klo = cs.ilo[lane & (KC - 1)], khi = cs.ihi[lane & (KC - 1)];
+        const bool hit = lane < KC && klo <= khi && khi >= wrow0 && klo <= wrow1;
+        uint32_t kmask = (uint32_t)__ballot(hit);
+        fill_weights(cs, x0, y0, wtab, [&](int k, int px, float w) { Wt[k][px] = w; });
+        fill_spectra(cs, nk, c0, g.nc, d.spectrum, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
-        for (int k = 0; k < nk; ++k) {
-            if (cs.ihi[k] < wrow0 || cs.ilo[k] > wrow1) continue;
+        if (kmask == 0u) continue;
+        while (kmask != 0u) {
+            const int k = __builtin_ctz(kmask);
+            kmask &= kmask - 1u;
             const float4* wr = reinterpret_cast<const float4*>(&Wt[k][pg * TILE]);
             const float4 sv = *reinterpret_cast<const float4*>(&St[k][cg * 4]);
 #pragma unroll
@@ -87,7 +91,11 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
                 }
             }
         }
-        __syncthreads();
+        // Clear this wave's 64-pixel band of W (read only by this wave) for the next fill.
+#pragma unroll
+        for (int k = 0; k < KC; k += 4)
+            *reinterpret_cast<float4*>(&Wt[k + (lane >> 4)][wave * 64 + (lane & 15) * 4]) =
+                make_float4(0.f, 0.f, 0.f, 0.f);
     }
 
     const int i = x0 + pg;

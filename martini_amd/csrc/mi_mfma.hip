@@ -5,17 +5,25 @@
 // per cell (16 x 16 pixels x 64 channels) as a block-sparse product of a
 // 256 x 32 W tile and a 32 x 64 S tile per chunk of 32 spatially sorted
 // particles.  Both operands are split into BF16 pairs x = hi + lo
-// (|x - hi - lo| <= 2^-16 |x|) and each 16 x 16 x 32 block takes three
-// v_mfma_f32_16x16x32_bf16 products -- hi*hi, hi*lo, lo*hi -- accumulated in
-// FP32 (the dropped lo*lo term is below 2^-16 of the product).
+// (|x - hi - lo| <= 2^-16 |x|) when they are written to LDS, and each
+// 16 x 16 x 32 block takes three v_mfma_f32_16x16x32_bf16 products -- hi*hi,
+// hi*lo, lo*hi -- accumulated in FP32 (the dropped lo*lo term is below 2^-16
+// of the product).
 //
 // Wave w owns tile rows 4w..4w+3 (M = the 16 pixels of a row) and all four
 // 16-channel blocks (N).  A (row, channel-block) product is skipped when no
 // footprint of the chunk touches the row or no spectral window touches the
-// block: the sparsity is skipped at block granularity, uniformly per wave.
-// W and S come from the same staging as the FP32 gather (mi_tile.h), laid
-// out k-contiguous in LDS so each lane loads its 8 K values in two 16-byte
-// reads.  Accumulation order is fixed: deterministic, no atomics.
+// block; the masks are wave-uniform, so the skips are scalar branches.
+//
+// LDS operand tiles hold 32 BF16 values (64 bytes) per pixel or channel, k
+// contiguous, with the four 16-byte k-groups of a row permuted by bits 2..3 of
+// the row index: the 16 rows a 16-lane group reads for one k-group then fall
+// in 16 distinct 16-byte bank slots.
+//
+// Per chunk there are two barriers: staging -> (W, S fill) -> products.  Each
+// wave clears the W rows it used right after reading them, so the next fill
+// finds them zero; S is fully rewritten by every fill.
+// Accumulation order is fixed: deterministic, no atomics.
 #include <hip/hip_runtime.h>
 
 #include "mi_device.h"
@@ -26,9 +34,8 @@ namespace mi {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef short shortx8 __attribute__((ext_vector_type(8)));
 
-constexpr int KPAD = KC + 4;  // row stride (floats) of the k-contiguous LDS tiles
+constexpr int KROW = KC * 2;  // bytes per LDS operand row (32 BF16)
 
 // Round-to-nearest-even BF16 bits of a finite float.
 __device__ inline uint32_t bf16_bits(float x) {
@@ -36,28 +43,23 @@ __device__ inline uint32_t bf16_bits(float x) {
     return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
 }
 
-// Splits 8 consecutive floats into BF16 hi and lo parts.
-__device__ inline void split8(const float* src, shortx8& hi, shortx8& lo) {
-    const float4 a = *reinterpret_cast<const float4*>(src);
-    const float4 b = *reinterpret_cast<const float4*>(src + 4);
-    const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        uint32_t h = bf16_bits(v[q]);
-        float r = v[q] - __uint_as_float(h << 16);
-        hi[q] = (short)h;
-        lo[q] = (short)bf16_bits(r);
-    }
+// Byte offset of element (row, k) in an operand tile.
+__device__ inline int op_offset(int row, int k) {
+    return row * KROW + ((((k >> 3) ^ (row >> 2)) & 3) << 4) + ((k & 7) << 1);
 }
 
-__device__ inline floatx4 mfma3(const shortx8& ah, const shortx8& al, const shortx8& bh,
-                                const shortx8& bl, floatx4 c) {
-    const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah), Al = __builtin_bit_cast(bf16x8, al);
-    const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl, c, 0, 0, 0);
-    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, c, 0, 0, 0);
-    return c;
+// Writes x as a BF16 pair at (row, k) of the hi and lo tiles.
+__device__ inline void put_split(char* hi, char* lo, int row, int k, float x) {
+    const uint32_t h = bf16_bits(x);
+    const uint32_t l = bf16_bits(x - __uint_as_float(h << 16));
+    const int o = op_offset(row, k);
+    *reinterpret_cast<uint16_t*>(hi + o) = (uint16_t)h;
+    *reinterpret_cast<uint16_t*>(lo + o) = (uint16_t)l;
+}
+
+// The 8 BF16 values of k-group kg of `row`.
+__device__ inline bf16x8 get8(const char* tile, int row, int kg) {
+    return *reinterpret_cast<const bf16x8*>(tile + row * KROW + (((kg ^ (row >> 2)) & 3) << 4));
 }
 
 __global__ __launch_bounds__(NT) void k_gather_mfma(
@@ -71,15 +73,22 @@ __global__ __launch_bounds__(NT) void k_gather_mfma(
     const CellPos cp = cell_pos(it.cell, g);
     const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
 
-    __shared__ __attribute__((aligned(16))) float Wp[TILE_PX][KPAD];  // [pixel][k]
-    __shared__ __attribute__((aligned(16))) float Sp[CBLOCK][KPAD];   // [channel][k]
+    __shared__ __attribute__((aligned(16))) char Ah[TILE_PX * KROW];  // W hi [pixel][k]
+    __shared__ __attribute__((aligned(16))) char Al[TILE_PX * KROW];  // W lo
+    __shared__ __attribute__((aligned(16))) char Bh[CBLOCK * KROW];   // S hi [channel][k]
+    __shared__ __attribute__((aligned(16))) char Bl[CBLOCK * KROW];   // S lo
     __shared__ ChunkState cs;
 
     const int t = threadIdx.x;
-    const int wave = t >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
     const int lane = t & 63;
-    const int m = lane & 15;  // A row / B column / D column of this lane
-    const int kg = lane >> 4; // K group of 8 (A, B); D row group of 4
+    const int m = lane & 15;   // A row / B column / D column of this lane
+    const int kg = lane >> 4;  // K group of 8 (A, B); D row group of 4
+
+    for (int e = t; e < TILE_PX * KROW / 16; e += NT) {
+        reinterpret_cast<uint4*>(Ah)[e] = make_uint4(0u, 0u, 0u, 0u);
+        reinterpret_cast<uint4*>(Al)[e] = make_uint4(0u, 0u, 0u, 0u);
+    }
 
     floatx4 acc[4][4];
 #pragma unroll
@@ -93,36 +102,46 @@ __global__ __launch_bounds__(NT) void k_gather_mfma(
     for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
         if (wave == 0) {
-            stage_chunk(next, t < nk, x0, y0, c0, g, cs);
+            stage_chunk(next, t < nk, x0, y0, c0, g, d, cs);
             const int nk2 = min(KC, it.end - kb - KC);
             if (t < nk2) next = sorted[kb + KC + t];
         }
-        for (int e = t; e < TILE_PX * KPAD / 4; e += NT)
-            reinterpret_cast<float4*>(&Wp[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-        fill_weights(cs, x0, y0, d, wtab, [&](int k, int px, float w) { Wp[px][k] = w; });
-        for (int e = t; e < CBLOCK * KC; e += NT) {
-            int cl = e / KC, k = e - cl * KC;
-            int c = c0 + cl;
-            Sp[cl][k] = (k < nk && c < g.nc) ? channel_flux(cs.p[k], c, d.spectrum) : 0.0f;
+        const uint32_t rows = (__builtin_amdgcn_readfirstlane(cs.rowmask) >> (wave * 4)) & 15u;
+        const uint32_t chans = __builtin_amdgcn_readfirstlane(cs.chanmask);
+        fill_weights(cs, x0, y0, wtab,
+                     [&](int k, int px, float w) { put_split(Ah, Al, px, k, w); });
+        fill_spectra(cs, nk, c0, g.nc, d.spectrum,
+                     [&](int k, int cl, float s) { put_split(Bh, Bl, cl, k, s); });
+        __syncthreads();
+        if (rows != 0u) {
+            bf16x8 bh[4], bl[4];
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                bh[nb] = get8(Bh, nb * 16 + m, kg);
+                bl[nb] = get8(Bl, nb * 16 + m, kg);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (!(rows & (1u << r))) continue;
+                const int row = (wave * 4 + r) * TILE + m;
+                const bf16x8 ah = get8(Ah, row, kg), al = get8(Al, row, kg);
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    if (!(chans & (1u << nb))) continue;
+                    floatx4 c = acc[r][nb];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[nb], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[nb], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[nb], c, 0, 0, 0);
+                    acc[r][nb] = c;
+                }
+                // Clear the row's 1 KiB in both W tiles (one 16-byte store per lane each);
+                // the reads above come first in this wave's LDS order.
+                const int zb = (wave * 4 + r) * TILE * KROW + lane * 16;
+                *reinterpret_cast<uint4*>(Ah + zb) = make_uint4(0u, 0u, 0u, 0u);
+                *reinterpret_cast<uint4*>(Al + zb) = make_uint4(0u, 0u, 0u, 0u);
+            }
         }
-        __syncthreads();
-        const uint32_t rows = cs.rowmask, chans = cs.chanmask;
-        shortx8 bh[4], bl[4];
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
-            if ((chans >> nb) & 1u) split8(&Sp[nb * 16 + m][kg * 8], bh[nb], bl[nb]);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = wave * 4 + r;
-            if (!((rows >> row) & 1u)) continue;
-            shortx8 ah, al;
-            split8(&Wp[row * TILE + m][kg * 8], ah, al);
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb)
-                if ((chans >> nb) & 1u) acc[r][nb] = mfma3(ah, al, bh[nb], bl[nb], acc[r][nb]);
-        }
-        __syncthreads();
     }
 
     // D layout of a 16x16 block: lane holds rows 4*kg..4*kg+3 (pixels j) of column m (channel).

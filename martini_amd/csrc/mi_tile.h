@@ -21,16 +21,20 @@ constexpr int NT = 256;   // threads per work group
 struct ChunkState {
     mi_particle p[KC];
     int ilo[KC], ihi[KC], jlo[KC], jhi[KC];
+    int wdiv[KC];       // ceil(2^16 / footprint width): (n * wdiv) >> 16 == n / width for n < 4096
     int off[KC + 1];    // exclusive scan of the per-particle pair counts
+    double fa[KC], fe[KC];   // WFactors of each particle (mi_device.h)
+    float fb[KC], ft[KC], fn[KC];
     uint32_t rowmask;   // tile rows (x offsets) touched by any footprint of the chunk
     uint32_t chanmask;  // 16-channel sub-blocks of the cell touched by any window
 };
 
 // Called by all 64 lanes of wave 0: lane t < KC holds particle t of the chunk
 // in `p` (valid when t < nk).  Clips footprints to the tile, scans the pair
-// counts and ORs the row and channel-block masks across the wave.
+// counts, ORs the row and channel-block masks across the wave and forms each
+// particle's weight factors.
 __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int y0, int c0,
-                                   const Grid& g, ChunkState& cs) {
+                                   const Grid& g, const mi_cube_desc& d, ChunkState& cs) {
     const int lane = threadIdx.x & 63;
     int cnt = 0, ilo = 1, ihi = 0, jlo = 1, jhi = 0;
     uint32_t rows = 0, chans = 0;
@@ -52,14 +56,14 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
     }
     int incl = cnt;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int v = __shfl_up(incl, dd, 64);
+        if (lane >= dd) incl += v;
     }
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        rows |= __shfl_xor(rows, d, 64);
-        chans |= __shfl_xor(chans, d, 64);
+    for (int dd = 32; dd > 0; dd >>= 1) {
+        rows |= __shfl_xor(rows, dd, 64);
+        chans |= __shfl_xor(chans, dd, 64);
     }
     if (lane < KC) {
         if (valid) cs.p[lane] = p;
@@ -67,7 +71,18 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
         cs.ihi[lane] = ihi;
         cs.jlo[lane] = jlo;
         cs.jhi[lane] = jhi;
+        const int width = max(jhi - jlo + 1, 1);
+        cs.wdiv[lane] = (65536 + width - 1) / width;
         cs.off[lane + 1] = incl;
+        if (cnt > 0) {
+            const int kind = kind_of(p.r_kind), gi = gidx_of(p.r_kind);
+            const WFactors f = weight_factors(kind, p.h, d.gauss_t[gi], d.gauss_inv_norm[gi]);
+            cs.fa[lane] = f.a;
+            cs.fe[lane] = f.e;
+            cs.fb[lane] = f.b;
+            cs.ft[lane] = f.t;
+            cs.fn[lane] = f.n;
+        }
     }
     if (lane == 0) {
         cs.off[0] = 0;
@@ -77,31 +92,49 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
 }
 
 // Evaluates W for every (particle, pixel) pair of the chunk and hands each
-// value to store(k, pixel-in-tile, w).
+// value to store(k, pixel-in-tile, w).  Pair q belongs to the particle k with
+// off[k] <= q < off[k+1] (binary search over the chunk's scan); its pixel is
+// row q' / width, column q' % width of the clipped footprint, q' = q - off[k]
+// (width <= 16 and q' < 256, where the reciprocal multiply is exact).
 template <class Store>
-__device__ inline void fill_weights(const ChunkState& cs, int x0, int y0, const mi_cube_desc& d,
+__device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                                     const float4* __restrict__ wtab, Store store) {
     const int total = cs.off[KC];
     for (int q = threadIdx.x; q < total; q += NT) {
-        int lo = 0, hi = KC - 1;  // find k with off[k] <= q < off[k+1]
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (cs.off[mid] <= q) lo = mid;
-            else hi = mid - 1;
-        }
-        const int k = lo;
-        const int width = cs.jhi[k] - cs.jlo[k] + 1;
+        int k = 0;
+#pragma unroll
+        for (int step = KC / 2; step > 0; step >>= 1)
+            if (cs.off[k + step] <= q) k += step;
         const int local = q - cs.off[k];
-        const int a = local / width;
-        const int b = local - a * width;
+        const int a = (local * cs.wdiv[k]) >> 16;
+        const int b = local - a * (cs.jhi[k] - cs.jlo[k] + 1);
         const int i = cs.ilo[k] + a, j = cs.jlo[k] + b;
         const mi_particle& p = cs.p[k];
         const double dx = (double)(p.ix - i) + (double)p.fx;
         const double dy = (double)(p.iy - j) + (double)p.fy;
-        const int gi = gidx_of(p.r_kind);
-        store(k, (i - x0) * TILE + (j - y0),
-              pixel_weight(kind_of(p.r_kind), dx, dy, p.h, d.gauss_t[gi], d.gauss_inv_norm[gi],
-                           wtab));
+        const WFactors f{cs.fa[k], cs.fe[k], cs.fb[k], cs.ft[k], cs.fn[k]};
+        store(k, (i - x0) * TILE + (j - y0), weight_at(kind_of(p.r_kind), dx, dy, f, wtab));
+    }
+}
+
+// Evaluates S for the chunk's KC particles x CBLOCK channels of the cell and
+// hands each value to store(k, channel-in-block, s); particles k >= nk and
+// channels beyond the cube give 0.  Thread t takes particle t % KC, so its
+// record fields are read once.
+template <class Store>
+__device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc, int spectrum,
+                                    Store store) {
+    const int k = threadIdx.x & (KC - 1);
+    const int cl0 = threadIdx.x / KC;
+    const mi_particle& p = cs.p[k];
+    const bool live = k < nk;
+    const int lo = live ? p.c_lo : 1, hi = live ? p.c_hi : 0;
+    const int k0 = p.k0;
+    const float fu = p.fu, s = p.s, amp = p.amp;
+#pragma unroll
+    for (int cl = cl0; cl < CBLOCK; cl += NT / KC) {
+        const int c = c0 + cl;
+        store(k, cl, c < nc ? line_flux(lo, hi, k0, fu, s, amp, c, spectrum) : 0.0f);
     }
 }
 
