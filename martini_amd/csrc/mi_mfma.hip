@@ -37,24 +37,18 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int KROW = KC * 2;  // bytes per LDS operand row (32 BF16)
 
-// Round-to-nearest-even BF16 bits of a finite float.
-__device__ inline uint32_t bf16_bits(float x) {
-    uint32_t u = __float_as_uint(x);
-    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-}
-
 // Byte offset of element (row, k) in an operand tile.
 __device__ inline int op_offset(int row, int k) {
     return row * KROW + ((((k >> 3) ^ (row >> 2)) & 3) << 4) + ((k & 7) << 1);
 }
 
-// Writes x as a BF16 pair at (row, k) of the hi and lo tiles.
+// Writes x as a BF16 pair (round to nearest even) at (row, k) of the hi and lo tiles.
 __device__ inline void put_split(char* hi, char* lo, int row, int k, float x) {
-    const uint32_t h = bf16_bits(x);
-    const uint32_t l = bf16_bits(x - __uint_as_float(h << 16));
+    const __bf16 h = (__bf16)x;
+    const __bf16 l = (__bf16)(x - (float)h);
     const int o = op_offset(row, k);
-    *reinterpret_cast<uint16_t*>(hi + o) = (uint16_t)h;
-    *reinterpret_cast<uint16_t*>(lo + o) = (uint16_t)l;
+    *reinterpret_cast<__bf16*>(hi + o) = h;
+    *reinterpret_cast<__bf16*>(lo + o) = l;
 }
 
 // The 8 BF16 values of k-group kg of `row`.
@@ -62,7 +56,9 @@ __device__ inline bf16x8 get8(const char* tile, int row, int kg) {
     return *reinterpret_cast<const bf16x8*>(tile + row * KROW + (((kg ^ (row >> 2)) & 3) << 4));
 }
 
-__global__ __launch_bounds__(NT) void k_gather_mfma(
+// Three waves per SIMD: the 64 accumulators then live in VGPRs and three work
+// groups share a CU (LDS 48 KB each).
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_gather_mfma(
     const mi_particle* __restrict__ sorted, const Item* __restrict__ items,
     const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
     float* __restrict__ cube, float* __restrict__ partials) {

@@ -23,6 +23,7 @@ struct ChunkState {
     int ilo[KC], ihi[KC], jlo[KC], jhi[KC];
     int wdiv[KC];       // ceil(2^16 / footprint width): (n * wdiv) >> 16 == n / width for n < 4096
     int off[KC + 1];    // exclusive scan of the per-particle pair counts
+    double px[KC], py[KC];   // particle position ix + fx, iy + fy (exact in FP64)
     double fa[KC], fe[KC];   // WFactors of each particle (mi_device.h)
     float fb[KC], ft[KC], fn[KC];
     uint32_t rowmask;   // tile rows (x offsets) touched by any footprint of the chunk
@@ -75,6 +76,8 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
         cs.wdiv[lane] = (65536 + width - 1) / width;
         cs.off[lane + 1] = incl;
         if (cnt > 0) {
+            cs.px[lane] = (double)p.ix + (double)p.fx;
+            cs.py[lane] = (double)p.iy + (double)p.fy;
             const int kind = kind_of(p.r_kind), gi = gidx_of(p.r_kind);
             const WFactors f = weight_factors(kind, p.h, d.gauss_t[gi], d.gauss_inv_norm[gi]);
             cs.fa[lane] = f.a;
@@ -93,27 +96,54 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
 
 // Evaluates W for every (particle, pixel) pair of the chunk and hands each
 // value to store(k, pixel-in-tile, w).  Pair q belongs to the particle k with
-// off[k] <= q < off[k+1] (binary search over the chunk's scan); its pixel is
-// row q' / width, column q' % width of the clipped footprint, q' = q - off[k]
-// (width <= 16 and q' < 256, where the reciprocal multiply is exact).
+// off[k] <= q < off[k+1]; its pixel is row q' / width, column q' % width of
+// the clipped footprint, q' = q - off[k] (width <= 16 and q' < 256, where the
+// reciprocal multiply is exact).  Thread t takes the contiguous pairs
+// [t*per, (t+1)*per): one binary search, then an incremental walk that keeps
+// the particle's values in registers until its pairs run out.
 template <class Store>
 __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                                     const float4* __restrict__ wtab, Store store) {
     const int total = cs.off[KC];
-    for (int q = threadIdx.x; q < total; q += NT) {
-        int k = 0;
+    const int per = (total + NT - 1) / NT;
+    int q = threadIdx.x * per;
+    const int qend = min(q + per, total);
+    if (q >= qend) return;
+    int k = 0;
 #pragma unroll
-        for (int step = KC / 2; step > 0; step >>= 1)
-            if (cs.off[k + step] <= q) k += step;
-        const int local = q - cs.off[k];
-        const int a = (local * cs.wdiv[k]) >> 16;
-        const int b = local - a * (cs.jhi[k] - cs.jlo[k] + 1);
-        const int i = cs.ilo[k] + a, j = cs.jlo[k] + b;
-        const mi_particle& p = cs.p[k];
-        const double dx = (double)(p.ix - i) + (double)p.fx;
-        const double dy = (double)(p.iy - j) + (double)p.fy;
-        const WFactors f{cs.fa[k], cs.fe[k], cs.fb[k], cs.ft[k], cs.fn[k]};
-        store(k, (i - x0) * TILE + (j - y0), weight_at(kind_of(p.r_kind), dx, dy, f, wtab));
+    for (int step = KC / 2; step > 0; step >>= 1)
+        if (cs.off[k + step] <= q) k += step;
+    int kend = cs.off[k + 1];
+    int width = cs.jhi[k] - cs.jlo[k] + 1;
+    const int local = q - cs.off[k];
+    int a = (local * cs.wdiv[k]) >> 16;
+    int b = local - a * width;
+    int ilo = cs.ilo[k], jlo = cs.jlo[k];
+    double px = cs.px[k], py = cs.py[k];
+    int kind = kind_of(cs.p[k].r_kind);
+    WFactors f{cs.fa[k], cs.fe[k], cs.fb[k], cs.ft[k], cs.fn[k]};
+    for (; q < qend; ++q) {
+        if (q >= kend) {  // next particle that has pairs
+            do {
+                ++k;
+                kend = cs.off[k + 1];
+            } while (q >= kend);
+            width = cs.jhi[k] - cs.jlo[k] + 1;
+            a = 0;
+            b = 0;
+            ilo = cs.ilo[k];
+            jlo = cs.jlo[k];
+            px = cs.px[k];
+            py = cs.py[k];
+            kind = kind_of(cs.p[k].r_kind);
+            f = WFactors{cs.fa[k], cs.fe[k], cs.fb[k], cs.ft[k], cs.fn[k]};
+        }
+        const int i = ilo + a, j = jlo + b;
+        store(k, (i - x0) * TILE + (j - y0), weight_at(kind, px - (double)i, py - (double)j, f, wtab));
+        if (++b == width) {
+            b = 0;
+            ++a;
+        }
     }
 }
 
