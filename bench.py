@@ -9,7 +9,11 @@ reference's square mask (martini/martini.py:271-274).  The reference's own
 CPU algorithm (oracle/cpu_baseline.py) is timed beside it on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--method plain|mfma] [--no-cpu-baseline]
+                    [--method plain|mfma] [--no-cpu-baseline] [--serving N]
+
+The serving leg (``--serving N`` requests, default 512) runs the
+continuous-batching service of martini_amd.serving on a mixed request stream
+and reports p50/p99 per-cube latency beside the throughput.
 """
 
 from __future__ import annotations
@@ -101,6 +105,55 @@ def time_gpu(rec, desc, n_pairs, method: str, steps: int, warmup: int):
     return ms, gather_ms, st, cube
 
 
+def serving_leg(n_requests: int, method: str, slots: int, load: float, seed: int = 5000) -> dict:
+    """The serving form on a request stream: capacity with every request
+    waiting at t = 0, then the same requests with Poisson arrivals at `load`
+    times that capacity.  Raw particle arrays are resident in HBM before the
+    stream starts; latency runs from arrival to the cube in pinned host memory."""
+    import dataclasses
+
+    import torch
+
+    from martini_amd import serving
+
+    specs = serving.request_stream(n_requests, seed=seed)
+    t = time.time()
+    sources = [serving.disc_on_device(s) for s in specs]
+    torch.cuda.synchronize()
+    log(f"[bench] serving: {n_requests} requests, {sum(s.n for s in specs):.3e} particles "
+        f"generated in HBM in {time.time() - t:.1f}s")
+    svc = serving.Service(slots=slots, method=method, keep_cubes=False)
+    svc.warm()
+    at0 = [dataclasses.replace(s, arrival=0.0) for s in specs]
+    svc.run(at0[:8], sources[:8])  # allocator and code warm-up
+    sat = serving.summarize(svc.run(at0, sources))
+    capacity = n_requests / sat["wall_s"]
+    rate = load * capacity
+    scale = specs[-1].arrival * rate / n_requests  # stretch the drawn gaps to the target rate
+    timed = [dataclasses.replace(s, arrival=s.arrival / scale) for s in specs]
+    res = serving.summarize(svc.run(timed, sources))
+    log(f"[bench] serving: capacity {capacity:.1f} req/s ({sat['updates_per_s']:.3e} updates/s "
+        f"saturated); at {rate:.1f} req/s p50 {res['p50_ms']:.2f} ms p99 {res['p99_ms']:.2f} ms")
+    return {
+        "requests": n_requests,
+        "method": method,
+        "slots": slots,
+        "capacity_req_per_s": capacity,
+        "saturated_updates_per_s": sat["updates_per_s"],
+        "offered_load": load,
+        "rate_req_per_s": rate,
+        "p50_ms": res["p50_ms"],
+        "p99_ms": res["p99_ms"],
+        "mean_ms": res["mean_ms"],
+        "queue_p50_ms": res["queue_p50_ms"],
+        "updates_per_s": res["updates_per_s"],
+        "particles": res["particles"],
+        "admission_waits": svc.admission_waits,
+        "mix": "N log-U[1e4,1e7], incl U(0,90), D U(3,30) Mpc, cubes {64^2x32..512^2x256}, "
+               "CubicSpline/WendlandC2, thermal",
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +163,13 @@ def main(argv=None) -> int:
     ap.add_argument("--method", default="plain", choices=["plain", "mfma"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="one device copy of this many MiB before timing (counter calibration)")
+    ap.add_argument("--serving", type=int, default=512,
+                    help="requests in the serving leg (0 skips it)")
+    ap.add_argument("--serving-method", default="mfma", choices=["plain", "mfma"])
+    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-load", type=float, default=0.8)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU "
@@ -122,6 +182,14 @@ def main(argv=None) -> int:
     U = updates_of(m)
     log(f"[bench] {args.workload}: {rec.size} particles, {n_pairs} cell pairs, U={U:.3e}, "
         f"host setup {time.time() - t:.1f}s")
+    if args.copy_probe_mb > 0:
+        import torch
+
+        a = torch.ones(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     ms, gather_ms, st, _ = time_gpu(rec, desc, n_pairs, args.method, args.steps, args.warmup)
     log(f"[bench] {args.method}: {ms:.3f} ms/step, gather {gather_ms:.3f} ms, items {st[2]}, "
         f"split items {st[3]}")
@@ -131,6 +199,10 @@ def main(argv=None) -> int:
     peak = PEAK_TFLOPS[args.method]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None}
+
+    srv = None
+    if args.serving > 0:
+        srv = serving_leg(args.serving, args.serving_method, args.serving_slots, args.serving_load)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -175,6 +247,7 @@ def main(argv=None) -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": srv,
     }
     print(json.dumps(out))
     return 0

@@ -170,6 +170,64 @@ int mi_kernel_weights(int kind, float gauss_t, float gauss_inv_norm, const float
 int mi_spectra(const mi_particle* d_particles, int64_t n, const mi_cube_desc* desc,
                float* d_out, void* stream);
 
+/*
+ * FP64 setup on the device.  An observation of one source: the source's
+ * placement (SPHSource, martini/sources/sph_source.py:171-326), the cube's
+ * WCS (DataCube, martini/datacube.py:426-562), the kernel tiers
+ * (_AdaptiveKernel, martini/sph_kernels.py:1207-1274; one tier for a fixed
+ * kernel) and the spectral model (martini/spectral_models.py).  Angles in
+ * degrees, distances Mpc, velocities km/s (m/s for the WCS values).
+ */
+typedef struct mi_observation {
+    double rotation[9];      /* source rotation applied to xyz and vxyz first (row-major) */
+    double sky_rotation[9];  /* R_z(ra) R_y(-dec) of the source position (row-major) */
+    double pointing[3];      /* unit vector towards (ra, dec) */
+    double distance_mpc, vpeculiar_kms, hubble_h;
+    double cube_ra_deg, cube_dec_deg, px_size_arcsec;
+    double crpix[3];         /* 1-based reference pixels of the padded cube */
+    double cdelt[3];         /* deg, deg, m/s (VRAD) or Hz (FREQ) */
+    double crval2;           /* spectral reference value, m/s or Hz */
+    double e0_ms, de_ms;     /* first velocity channel edge and signed width, m/s */
+    double dv_max_kms;       /* max |velocity channel width|, km/s (spectral pruning) */
+    double sigma_kms;        /* fixed line width, km/s; < 0 selects thermal sqrt(k_B T / m_p) */
+    double out_scale;        /* multiplies amp (1/px^2 gives Jy/arcsec^2) */
+    double tier_rescale[3], tier_size[3];  /* _rescale and size_in_fwhm per tier */
+    double tier_min[3], tier_max[3];       /* tier valid when min <= h * rescale <= max */
+    int32_t tier_kind[3], tier_gidx[3];    /* MI_KERNEL_* and Gaussian table index per tier */
+    int32_t n_tiers;         /* 1: fixed kernel; 2-3: first valid tier wins */
+    int32_t nx, ny, padx, pady, nc;
+    int32_t spectrum;        /* MI_SPECTRUM_* */
+    int32_t freq_channels;   /* spectral axis in frequency (radio convention) */
+    int32_t prune;           /* apply _BaseMartini._prune_particles (martini/martini.py:168-241) */
+} mi_observation;
+
+/* Flags reported by mi_prepare (counts[2]). */
+#define MI_PREP_INVALID_KERNEL 1u /* some kept particle fails kernel validation */
+#define MI_PREP_COORD_LIMIT 2u    /* some pixel coordinate beyond 2^21 */
+
+/* Workspace bytes for mi_prepare over n particles. */
+int mi_prepare_workspace_size(int64_t n, size_t* bytes);
+
+/*
+ * Sky projection, pixel and channel coordinates, smoothing lengths, kernel
+ * tier, pruning and record packing of n particles, all in FP64 on the
+ * device (replaces Martini.__init__'s setup, martini/martini.py:118-151, and
+ * the record packing that mi_insert consumes).  Inputs are device arrays:
+ * d_xyz, d_vxyz (n x 3, kpc and km/s, row-major), d_mHI (Msun), d_hsm (kpc),
+ * d_T (K; may be null when sigma_kms >= 0), d_edges (nc + 1 velocity channel
+ * edges in m/s; needed for the Dirac spectrum, may be null otherwise).
+ * Kept particles are written to d_out in input order (d_out holds n records).
+ * d_counts (device, 4 x uint64): kept particles, their cell pairs (the
+ * n_pairs that mi_insert_workspace_size and mi_insert take), MI_PREP_* flags,
+ * and the pixels of the reference's square masks (martini/martini.py:271-274)
+ * summed over kept particles (times nc: the particle-voxel updates).
+ * Asynchronous.
+ */
+int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
+               const double* d_hsm, const double* d_T, const double* d_edges, int64_t n,
+               const mi_observation* obs, mi_particle* d_out, uint64_t* d_counts,
+               void* d_workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,29 @@ class CubeDesc(ctypes.Structure):
                 ("gauss_inv_norm", ctypes.c_float * 4)]
 
 
+class Observation(ctypes.Structure):
+    """struct mi_observation (device FP64 setup of one source)."""
+
+    _d3 = ctypes.c_double * 3
+    _fields_ = [("rotation", ctypes.c_double * 9), ("sky_rotation", ctypes.c_double * 9),
+                ("pointing", _d3), ("distance_mpc", ctypes.c_double),
+                ("vpeculiar_kms", ctypes.c_double), ("hubble_h", ctypes.c_double),
+                ("cube_ra_deg", ctypes.c_double), ("cube_dec_deg", ctypes.c_double),
+                ("px_size_arcsec", ctypes.c_double), ("crpix", _d3), ("cdelt", _d3),
+                ("crval2", ctypes.c_double), ("e0_ms", ctypes.c_double), ("de_ms", ctypes.c_double),
+                ("dv_max_kms", ctypes.c_double), ("sigma_kms", ctypes.c_double),
+                ("out_scale", ctypes.c_double), ("tier_rescale", _d3), ("tier_size", _d3),
+                ("tier_min", _d3), ("tier_max", _d3), ("tier_kind", ctypes.c_int32 * 3),
+                ("tier_gidx", ctypes.c_int32 * 3), ("n_tiers", ctypes.c_int32),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("padx", ctypes.c_int32),
+                ("pady", ctypes.c_int32), ("nc", ctypes.c_int32), ("spectrum", ctypes.c_int32),
+                ("freq_channels", ctypes.c_int32), ("prune", ctypes.c_int32)]
+
+
+PREP_INVALID_KERNEL = 1
+PREP_COORD_LIMIT = 2
+
+
 class MartiniLibError(RuntimeError):
     """A C-ABI call returned an error code."""
 
@@ -88,6 +111,11 @@ def lib() -> ctypes.CDLL:
     L.mi_kernel_weights.argtypes = [ctypes.c_int, f32, f32, vp, vp, vp, vp, i64, vp]
     L.mi_spectra.restype = ctypes.c_int
     L.mi_spectra.argtypes = [vp, i64, ctypes.POINTER(CubeDesc), vp, vp]
+    L.mi_prepare_workspace_size.restype = ctypes.c_int
+    L.mi_prepare_workspace_size.argtypes = [i64, ctypes.POINTER(ctypes.c_size_t)]
+    L.mi_prepare.restype = ctypes.c_int
+    L.mi_prepare.argtypes = [vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(Observation), vp, vp,
+                             vp, ctypes.c_size_t, vp]
     _lib = L
     return L
 
@@ -203,3 +231,23 @@ def spectra(records: np.ndarray, desc: CubeDesc) -> np.ndarray:
     check(lib().mi_spectra(ctypes.c_void_p(rec.data_ptr()), n, ctypes.byref(desc),
                            ctypes.c_void_p(out.data_ptr()), stream_ptr()), "mi_spectra")
     return out.cpu().numpy()
+
+
+def prepare_workspace_bytes(n: int) -> int:
+    out = ctypes.c_size_t(0)
+    check(lib().mi_prepare_workspace_size(int(n), ctypes.byref(out)), "mi_prepare_workspace_size")
+    return int(out.value)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def prepare(d_xyz, d_vxyz, d_mHI, d_hsm, d_T, d_edges, n: int, obs: Observation, d_out,
+            d_counts, d_ws, stream=None) -> None:
+    """Asynchronous device FP64 setup (mi_prepare); d_counts receives
+    (kept, cell pairs, flags, mask pixels) as four uint64."""
+    check(lib().mi_prepare(_ptr(d_xyz), _ptr(d_vxyz), _ptr(d_mHI), _ptr(d_hsm), _ptr(d_T),
+                           _ptr(d_edges), int(n), ctypes.byref(obs), _ptr(d_out), _ptr(d_counts),
+                           _ptr(d_ws), d_ws.numel() * d_ws.element_size(), stream_ptr(stream)),
+          "mi_prepare")

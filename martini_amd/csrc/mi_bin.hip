@@ -51,7 +51,6 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     L->vals = take(4 * np);
     L->keys2 = take(4 * np);
     L->vals2 = take(4 * np);
-    L->sorted = take(sizeof(mi_particle) * np);
     L->cell_start = take(4 * (size_t)g.ncells);
     L->cell_end = take(4 * (size_t)g.ncells);
     L->item_counts = take(4 * nc1);
@@ -141,19 +140,6 @@ __global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap
             }
         }
     }
-}
-
-// Copies the records into cell order so each work item reads a contiguous
-// stretch of them.  After an overflow the pair buffers hold no valid indices
-// and nothing is read.
-__global__ void k_permute(const mi_particle* __restrict__ P, int64_t n,
-                          const uint32_t* __restrict__ order, int64_t np,
-                          const uint32_t* __restrict__ status, mi_particle* __restrict__ sorted) {
-    if (status[ST_OVERFLOW]) return;
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
-    const uint32_t k = order[q];
-    if ((int64_t)k < n) sorted[q] = P[k];
 }
 
 __global__ void k_fill_wtab(float4* __restrict__ tab) { fill_wtab(tab); }
@@ -256,9 +242,6 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
                                                  g.cell_bits + KEY_LOCAL_BITS, s);
         if (e != hipSuccess) return e;
     }
-    if (n > 0 && n_pairs > 0)
-        hipLaunchKernelGGL(k_permute, dim3(blocks_for(n_pairs, T)), dim3(T), 0, s, d_p, n, vals2,
-                           n_pairs, status, (mi_particle*)(ws + L.sorted));
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cstart, g.ncells, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cend, g.ncells, 0u);
     if (n_pairs > 0)

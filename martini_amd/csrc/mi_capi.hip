@@ -161,4 +161,54 @@ int mi_spectra(const mi_particle* d_particles, int64_t n, const mi_cube_desc* de
     return MI_OK;
 }
 
+int mi_prepare_workspace_size(int64_t n, size_t* bytes) {
+    if (n < 0 || !bytes) return MI_ERR_ARGUMENT;
+    if (n >= ((int64_t)1 << 31)) return MI_ERR_LIMIT;
+    PrepLayout L;
+    if (prep_layout(n, &L) != hipSuccess) return MI_ERR_DEVICE;
+    *bytes = L.total;
+    return MI_OK;
+}
+
+static bool observation_ok(const mi_observation* o) {
+    if (!o) return false;
+    if (o->nx <= 0 || o->ny <= 0 || o->nc <= 0 || o->padx < 0 || o->pady < 0) return false;
+    if (o->spectrum != MI_SPECTRUM_GAUSS && o->spectrum != MI_SPECTRUM_DIRAC) return false;
+    if (o->n_tiers < 1 || o->n_tiers > 3) return false;
+    for (int k = 0; k < o->n_tiers; ++k) {
+        if (o->tier_kind[k] < MI_KERNEL_CUBIC || o->tier_kind[k] > MI_KERNEL_DIRAC) return false;
+        if (o->tier_gidx[k] < 0 || o->tier_gidx[k] > 3) return false;
+    }
+    if (!(o->px_size_arcsec > 0.0) || !(o->dv_max_kms > 0.0) || o->de_ms == 0.0) return false;
+    return true;
+}
+
+int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
+               const double* d_hsm, const double* d_T, const double* d_edges, int64_t n,
+               const mi_observation* obs, mi_particle* d_out, uint64_t* d_counts,
+               void* d_workspace, size_t workspace_bytes, void* stream) {
+    if (!observation_ok(obs) || n < 0 || !d_counts) return MI_ERR_ARGUMENT;
+    if (n > 0 && (!d_xyz || !d_vxyz || !d_mHI || !d_hsm || !d_out || !d_workspace))
+        return MI_ERR_ARGUMENT;
+    if (n > 0 && obs->spectrum == MI_SPECTRUM_GAUSS && obs->sigma_kms < 0.0 && !d_T)
+        return MI_ERR_ARGUMENT;
+    if (n > 0 && obs->spectrum == MI_SPECTRUM_DIRAC && !d_edges) return MI_ERR_ARGUMENT;
+    if (n >= ((int64_t)1 << 31)) return MI_ERR_LIMIT;
+    mi_cube_desc d{};
+    d.nx = obs->nx + 2 * obs->padx;
+    d.ny = obs->ny + 2 * obs->pady;
+    d.nc = obs->nc;
+    d.spectrum = obs->spectrum;
+    Grid g = make_grid(d);
+    if (check_limits(g, n, 0) != MI_OK) return MI_ERR_LIMIT;
+    PrepLayout L;
+    if (prep_layout(n, &L) != hipSuccess) return MI_ERR_DEVICE;
+    if (workspace_bytes < L.total) return MI_ERR_WORKSPACE;
+    if (prepare_particles(d_xyz, d_vxyz, d_mHI, d_hsm, d_T, d_edges, n, *obs, g, d_out,
+                          (unsigned long long*)d_counts, (char*)d_workspace, L,
+                          (hipStream_t)stream) != hipSuccess)
+        return MI_ERR_DEVICE;
+    return MI_OK;
+}
+
 }  // extern "C"

@@ -24,7 +24,8 @@
 namespace mi {
 
 __global__ __launch_bounds__(NT) void k_gather_plain(
-    const mi_particle* __restrict__ sorted, const Item* __restrict__ items,
+    const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
+    const Item* __restrict__ items,
     const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
     float* __restrict__ cube, float* __restrict__ partials) {
     if (status[ST_OVERFLOW]) return;
@@ -55,14 +56,14 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
         for (int cc = 0; cc < 4; ++cc) acc[jj][cc] = 0.0f;
 
     mi_particle next;  // wave 0: the record this lane stages next
-    if (wave == 0 && t < min(KC, it.end - it.begin)) next = sorted[it.begin + t];
+    if (wave == 0 && t < min(KC, it.end - it.begin)) next = P[order[it.begin + t]];
 
     for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
         if (wave == 0) {
             stage_chunk(next, t < nk, x0, y0, c0, g, d, cs);
             const int nk2 = min(KC, it.end - kb - KC);
-            if (t < nk2) next = sorted[kb + KC + t];
+            if (t < nk2) next = P[order[kb + KC + t]];
         }
         __syncthreads();
         // Particles whose footprint meets this wave's 4 rows (wave-uniform).
@@ -193,7 +194,7 @@ hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_des
                         const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_plain, dim3((unsigned)L.items_max), dim3(NT), 0, s,
-                       (const mi_particle*)(ws + L.sorted), (const Item*)(ws + L.items),
+                       d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
                        (const uint32_t*)(ws + L.status), (const float4*)(ws + L.wtab), g, d,
                        d_cube, (float*)(ws + L.partials));
     return hipGetLastError();

@@ -41,8 +41,7 @@ __host__ __device__ inline uint32_t cells_of(const mi_particle& p, const Grid& g
 // Byte offsets of the workspace regions (each 256-byte aligned).
 struct Layout {
     size_t counts, offsets;            // uint32[n] each
-    size_t keys, vals, keys2, vals2;   // uint32[n_pairs] each
-    size_t sorted;                     // mi_particle[n_pairs]: records in cell order
+    size_t keys, vals, keys2, vals2;   // uint32[n_pairs] each; vals2: particle index in cell order
     size_t cell_start, cell_end;       // uint32[ncells] each
     size_t item_counts, item_off;      // uint32[ncells + 1] each
     size_t multi_counts, multi_off;    // uint32[ncells + 1] each
@@ -78,6 +77,17 @@ hipError_t launch_weights(int kind, float gt, float ginv, const float* dx, const
                           const float* h, float* out, int64_t n, hipStream_t s);
 hipError_t launch_spectra(const mi_particle* d_p, int64_t n, const mi_cube_desc& d, float* out,
                           hipStream_t s);
+
+// Device FP64 setup (mi_prepare.hip): workspace layout and launcher.
+struct PrepLayout {
+    size_t keep, off, temp, temp_bytes, total;
+};
+hipError_t prep_layout(int64_t n, PrepLayout* L);
+hipError_t prepare_particles(const double* xyz, const double* vxyz, const double* mHI,
+                             const double* hsm, const double* T, const double* edges, int64_t n,
+                             const mi_observation& o, const Grid& g, mi_particle* out,
+                             unsigned long long* counts, char* ws, const PrepLayout& L,
+                             hipStream_t s);
 
 // One work item: particles [begin, end) of the sorted pair list belonging to
 // one cell; slot >= 0 names its partial tile, -1 means it writes the cube.

@@ -59,7 +59,8 @@ __device__ inline bf16x8 get8(const char* tile, int row, int kg) {
 // Three waves per SIMD: the 64 accumulators then live in VGPRs and three work
 // groups share a CU (LDS 48 KB each).
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_gather_mfma(
-    const mi_particle* __restrict__ sorted, const Item* __restrict__ items,
+    const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
+    const Item* __restrict__ items,
     const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
     float* __restrict__ cube, float* __restrict__ partials) {
     if (status[ST_OVERFLOW]) return;
@@ -93,14 +94,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         for (int nb = 0; nb < 4; ++nb) acc[r][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     mi_particle next;  // wave 0: the record this lane stages next
-    if (wave == 0 && t < min(KC, it.end - it.begin)) next = sorted[it.begin + t];
+    if (wave == 0 && t < min(KC, it.end - it.begin)) next = P[order[it.begin + t]];
 
     for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
         if (wave == 0) {
             stage_chunk(next, t < nk, x0, y0, c0, g, d, cs);
             const int nk2 = min(KC, it.end - kb - KC);
-            if (t < nk2) next = sorted[kb + KC + t];
+            if (t < nk2) next = P[order[kb + KC + t]];
         }
         __syncthreads();
         const uint32_t rows = (__builtin_amdgcn_readfirstlane(cs.rowmask) >> (wave * 4)) & 15u;
@@ -167,7 +168,7 @@ hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc
                        const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather_mfma, dim3((unsigned)L.items_max), dim3(NT), 0, s,
-                       (const mi_particle*)(ws + L.sorted), (const Item*)(ws + L.items),
+                       d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
                        (const uint32_t*)(ws + L.status), (const float4*)(ws + L.wtab), g, d,
                        d_cube, (float*)(ws + L.partials));
     return hipGetLastError();

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from martini_amd import synthetic
-from tests.helpers import FLUX_TOL, VOXEL_TOL, gpu_cube, oracle_cube, parity
+from oracle import martini_oracle as O
+from tests.helpers import FLUX_TOL, VOXEL_TOL, gpu_cube, oracle_cube, oracle_kernel, parity
 
 pytestmark = pytest.mark.gpu
 
@@ -41,3 +42,56 @@ def test_deterministic(method):
     a, _ = gpu_cube(w, method=method)
     b, _ = gpu_cube(w, method=method)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("method", ["plain", "mfma"])
+def test_hetero_smoothing_subsample(method):
+    """The heterogeneous-smoothing configuration (h 0.05-50 px, T 1e2 / 1e4 K,
+    1024^2 x 64) on a 1e5-particle subsample: tiny Dirac-fallback particles
+    and footprints spanning ~100 tiles in one launch."""
+    w = synthetic.WORKLOADS["hetero3e6_1024px"]
+    n = 100_000
+    ref, P = oracle_cube(w, n=n)
+    gpu, m = gpu_cube(w, n=n, method=method)
+    voxel, flux = parity(gpu, ref)
+    print(f"hetero n={n} {method}: voxel {voxel:.3e} flux {flux:.3e} "
+          f"r_max {int(P.sm_ranges.max())}")
+    assert voxel <= VOXEL_TOL
+    assert flux <= FLUX_TOL
+
+
+def _kernel_cases():
+    from martini_amd.sph_kernels import (CubicSplineKernel, DiracDeltaKernel, GaussianKernel,
+                                         QuarticSplineKernel, WendlandC6Kernel)
+    from martini_amd.spectral_models import DiracDeltaSpectrum, GaussianSpectrum
+
+    return {
+        "wc6": (WendlandC6Kernel, GaussianSpectrum, oracle_kernel("wc6"), "gauss", False),
+        "quartic": (QuarticSplineKernel, GaussianSpectrum, oracle_kernel("quartic"), "gauss", False),
+        "gauss_t3": (lambda: GaussianKernel(truncate=3.0), GaussianSpectrum,
+                     oracle_kernel("gauss", truncate=3.0), "gauss", False),
+        "dirac_kernel": (DiracDeltaKernel, GaussianSpectrum, O.KernelSpec("dirac"), "gauss", True),
+        "dirac_spectrum": (CubicSplineKernel, DiracDeltaSpectrum, oracle_kernel("cubic"), "dirac",
+                           False),
+    }
+
+
+@pytest.mark.parametrize("case", ["wc6", "quartic", "gauss_t3", "dirac_kernel", "dirac_spectrum"])
+@pytest.mark.parametrize("method", ["plain", "mfma"])
+def test_kernel_and_spectrum_kinds(case, method):
+    """Every kernel and spectrum kind through the full path, on the 64 px
+    disc with 4000 particles."""
+    from martini_amd import Martini
+
+    make_k, make_s, oracle_k, oracle_s, skip = _kernel_cases()[case]
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    n = 4000
+    ref, P = oracle_cube(w, n=n, kernel=oracle_k, spectrum=oracle_s)
+    spec = make_s(sigma="thermal") if oracle_s == "gauss" else make_s()
+    src, dc, k, s = synthetic.build(w, n, kernel=make_k(), spectrum=spec)
+    m = Martini(source=src, datacube=dc, sph_kernel=k, spectral_model=s, quiet=True)
+    m.insert_source_in_cube(skip_validation=skip, method=method)
+    voxel, flux = parity(m.datacube._array, ref)
+    print(f"{case} {method}: voxel {voxel:.3e} flux {flux:.3e}")
+    assert voxel <= VOXEL_TOL
+    assert flux <= FLUX_TOL

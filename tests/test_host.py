@@ -255,3 +255,44 @@ def test_dirac_spectrum_pruning_and_records():
     width = rec["c_hi"] - rec["c_lo"] + 1
     assert np.all((width >= 0) & (width <= 2))
     assert desc.spectrum == _lib.SPECTRUM_DIRAC
+
+
+def test_observation_struct_matches_host_objects():
+    from martini_amd.device_setup import observation
+
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    src, dc, k, s = synthetic.build(w)
+    o, desc = observation(distance=w.distance, datacube=dc, sph_kernel=k, spectral_model=s,
+                          rotation=w.rotation)
+    assert list(o.crpix) == list(dc.crpix) and list(o.cdelt) == list(dc.cdelt)
+    e = dc.velocity_channel_edges
+    assert o.e0_ms == e[0] and o.de_ms == (e[-1] - e[0]) / dc.n_channels
+    assert np.allclose(np.array(o.rotation).reshape(3, 3), w.rotation.as_matrix(), atol=0)
+    assert o.n_tiers == 3 and list(o.tier_kind) == [_lib.KERNEL_CUBIC, _lib.KERNEL_DIRAC,
+                                                    _lib.KERNEL_GAUSS]
+    for i, K in enumerate(k.kernels):
+        assert o.tier_rescale[i] == K._rescale and o.tier_size[i] == K.size_in_fwhm
+    assert o.tier_min[0] == 1.16 and o.tier_max[1] == 0.5 and o.tier_min[2] == 0.336
+    assert o.sigma_kms < 0  # thermal
+    assert (desc.nx, desc.ny, desc.nc) == (64, 64, 32)
+    assert abs(desc.gauss_t[0] - 6.0) < 1e-6
+
+
+def test_prepare_argument_errors():
+    from martini_amd.device_setup import observation
+
+    L = _lib.lib()
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    src, dc, k, s = synthetic.build(w)
+    o, _ = observation(distance=w.distance, datacube=dc, sph_kernel=k, spectral_model=s)
+    args = [None] * 6 + [0, None, None, None, None, 0, None]
+    assert L.mi_prepare(*args) == -1  # no observation
+    args[7] = ctypes.byref(o)
+    assert L.mi_prepare(*args) == -1  # no counts buffer
+    out = ctypes.c_size_t(0)
+    assert L.mi_prepare_workspace_size(-1, ctypes.byref(out)) == -1
+    o.n_tiers = 0
+    args[9] = ctypes.c_void_p(8)
+    assert L.mi_prepare(*args) == -1
+    o.n_tiers, o.spectrum = 3, 7
+    assert L.mi_prepare(*args) == -1
