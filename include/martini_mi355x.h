@@ -199,6 +199,9 @@ typedef struct mi_observation {
     int32_t spectrum;        /* MI_SPECTRUM_* */
     int32_t freq_channels;   /* spectral axis in frequency (radio convention) */
     int32_t prune;           /* apply _BaseMartini._prune_particles (martini/martini.py:168-241) */
+    int32_t channel_offset;  /* records address channels channel_offset + [0, nc) of a cube */
+    int32_t nc_total;        /* ... with nc_total channels (0: nc); several sources then share
+                                one insertion, each in its own channel range */
 } mi_observation;
 
 /* Flags reported by mi_prepare (counts[2]). */

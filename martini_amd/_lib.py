@@ -56,7 +56,8 @@ class Observation(ctypes.Structure):
                 ("tier_gidx", ctypes.c_int32 * 3), ("n_tiers", ctypes.c_int32),
                 ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("padx", ctypes.c_int32),
                 ("pady", ctypes.c_int32), ("nc", ctypes.c_int32), ("spectrum", ctypes.c_int32),
-                ("freq_channels", ctypes.c_int32), ("prune", ctypes.c_int32)]
+                ("freq_channels", ctypes.c_int32), ("prune", ctypes.c_int32),
+                ("channel_offset", ctypes.c_int32), ("nc_total", ctypes.c_int32)]
 
 
 PREP_INVALID_KERNEL = 1

@@ -180,6 +180,8 @@ static bool observation_ok(const mi_observation* o) {
         if (o->tier_gidx[k] < 0 || o->tier_gidx[k] > 3) return false;
     }
     if (!(o->px_size_arcsec > 0.0) || !(o->dv_max_kms > 0.0) || o->de_ms == 0.0) return false;
+    const int nct = o->nc_total > 0 ? o->nc_total : o->nc;
+    if (o->channel_offset < 0 || o->channel_offset + o->nc > nct) return false;
     return true;
 }
 
@@ -197,7 +199,7 @@ int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
     mi_cube_desc d{};
     d.nx = obs->nx + 2 * obs->padx;
     d.ny = obs->ny + 2 * obs->pady;
-    d.nc = obs->nc;
+    d.nc = obs->nc_total > 0 ? obs->nc_total : obs->nc;
     d.spectrum = obs->spectrum;
     Grid g = make_grid(d);
     if (check_limits(g, n, 0) != MI_OK) return MI_ERR_LIMIT;

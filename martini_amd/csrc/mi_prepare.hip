@@ -201,9 +201,11 @@ __device__ Setup setup_particle(int64_t i, const double* __restrict__ xyz,
         c_lo = 1.0;
         c_hi = 0.0;
     }
-    p.c_lo = (int32_t)c_lo;
-    p.c_hi = (int32_t)c_hi;
-    p.k0 = (int32_t)fmin(fmax(k0, -1073741824.0), 1073741824.0);
+    // a shared cube addresses this source's channels from channel_offset on
+    const bool empty = c_lo > c_hi;
+    p.c_lo = (int32_t)c_lo + (empty ? 0 : o.channel_offset);
+    p.c_hi = (int32_t)c_hi + (empty ? 0 : o.channel_offset);
+    p.k0 = (int32_t)fmin(fmax(k0, -1073741824.0), 1073741824.0) + o.channel_offset;
     p.fu = (float)fu;
     p.s = (float)s;
     p.amp = (float)amp;

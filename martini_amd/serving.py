@@ -16,10 +16,16 @@ one job).  Every request runs:
   3. ``mi_bin`` + ``mi_gather`` into a zeroed cube, and the cube's copy to
      pinned host memory.
 
+Small requests that wait together and share a cube shape are fused: their
+records address consecutive channel ranges of one stacked cube, so a single
+bin and gather serve them all (``Service`` fuse_max_n / fuse_max_jobs).
+
 Latency is measured from the request's arrival to the host seeing its cube.
-Work is per request (no state shared between requests on the device), and
-the kernels' summation order is fixed, so a request's cube is the same
-whether it runs alone or in the mix; ``tests/test_serving.py`` checks this.
+A request inserted on its own gets exactly the cube it gets alone (no state
+is shared between insertions and the kernels' summation order is fixed); a
+fused request's particles are chunked with its neighbours', which changes
+only the FP32 rounding (within 1e-5 of the peak).  ``tests/test_serving.py``
+checks both.
 
 Requests are independent ``Martini`` insertions: the per-request semantics
 of ``Martini(...)`` + ``insert_source_in_cube`` (martini/martini.py:827-861)
@@ -165,6 +171,7 @@ class Result:
     kept: int = 0
     pairs: int = 0
     updates: int = 0        # particle-voxel updates, reference mask semantics
+    group: int = 1          # requests inserted together with this one (fusion)
     cube: np.ndarray | None = None
 
     @property
@@ -173,30 +180,46 @@ class Result:
 
 
 @dataclass
-class _Job:
+class _Member:
     spec: RequestSpec
     src: DeviceSource
     res: Result
     obs: object = None
     desc: object = None
+
+
+@dataclass
+class _Job:
+    """One insertion on the GPU: a single request, or several small requests of
+    one cube shape stacked along the channel axis of a shared cube."""
+
+    members: list
     slot: int = -1
     phase: str = "wait"
     reserved: int = 0
     bufs: dict = field(default_factory=dict)
     event: object = None
+    desc: object = None  # descriptor of the (possibly stacked) cube
 
 
 class Service:
     """Continuous-batching insertion service on one GPU.
 
-    slots: requests on the GPU at once (one HIP stream each, default priority).
-    hbm_budget: bytes of HBM for the requests' working sets (records, setup and
-    insertion workspaces, cubes; the raw particle arrays are the requests' own).
+    slots: insertions on the GPU at once (one HIP stream each, default priority).
+    hbm_budget: bytes of HBM for the working sets (records, setup and insertion
+    workspaces, cubes; the raw particle arrays are the requests' own).
     method: 'plain' or 'mfma'.
+    fuse_max_n, fuse_max_jobs: requests of at most fuse_max_n particles that
+    wait together and share a cube shape are inserted together, up to
+    fuse_max_jobs of them, their cubes stacked along the channel axis (each
+    record's channel window lies inside its own request's channels, so the
+    binning and gather kernels run unchanged on the stacked cube).  One bin and
+    one gather then serve all of them.  0 disables fusion.
     """
 
     def __init__(self, slots: int = 4, hbm_budget: int = 64 << 30, method: str = "mfma",
-                 keep_cubes: bool = True) -> None:
+                 keep_cubes: bool = True, fuse_max_n: int = 100_000,
+                 fuse_max_jobs: int = 16) -> None:
         import torch
 
         self.torch = torch
@@ -206,11 +229,14 @@ class Service:
         self.budget = int(hbm_budget)
         self.method = method
         self.keep_cubes = keep_cubes
+        self.fuse_max_n = int(fuse_max_n)
+        self.fuse_max_jobs = max(1, int(fuse_max_jobs))
         self.in_use = 0
-        self.pinned_counts = [torch.zeros(4, dtype=torch.int64).pin_memory()
+        self.pinned_counts = [torch.zeros((self.fuse_max_jobs, 4), dtype=torch.int64).pin_memory()
                               for _ in range(self.slots)]
         self._pinned_cubes: dict[int, list] = {}
         self.admission_waits = 0
+        self.fused_launches = 0
 
     def warm(self, shapes=SHAPES) -> None:
         """Pins one host cube buffer per slot and shape class up front, so that
@@ -237,42 +263,82 @@ class Service:
             return True
         return False
 
+    # ---------------------------------------------------------- grouping
+    def _take_group(self, waiting: deque) -> list:
+        """The next job's members: the head request, plus (when it is small)
+        the small requests waiting behind it with the same cube and tables."""
+        head = waiting.popleft()
+        if self.fuse_max_n <= 0 or head.spec.n > self.fuse_max_n or self.fuse_max_jobs < 2:
+            return [head]
+        head.obs, head.desc, _ = observe(head.spec)
+        group, rest = [head], deque()
+        while waiting:
+            m = waiting.popleft()
+            s, h = m.spec, head.spec
+            if (len(group) < self.fuse_max_jobs and s.n <= self.fuse_max_n
+                    and (s.n_px, s.n_channels) == (h.n_px, h.n_channels)):
+                m.obs, m.desc, _ = observe(s)
+                if (m.desc.spectrum == head.desc.spectrum
+                        and tuple(m.desc.gauss_t) == tuple(head.desc.gauss_t)
+                        and tuple(m.desc.gauss_inv_norm) == tuple(head.desc.gauss_inv_norm)):
+                    group.append(m)
+                    continue
+            rest.append(m)
+        waiting.extend(rest)
+        return group
+
     # ---------------------------------------------------------- stages
     def _launch_prepare(self, job: _Job, now: float) -> bool:
         torch = self.torch
-        src = job.src
-        prep_ws = _lib.prepare_workspace_bytes(src.n)
-        need = 48 * max(src.n, 1) + prep_ws + 256
+        B = len(job.members)
+        need = sum(48 * max(m.src.n, 1) + _lib.prepare_workspace_bytes(m.src.n) + 256
+                   for m in job.members)
         if not self._reserve(need, force=self.in_use == 0):
             return False
         job.reserved = need
-        job.obs, job.desc, dc = observe(job.spec)
         s = self.streams[job.slot]
+        counts = torch.empty((B, 4), dtype=torch.int64, device=self.dev)
+        outs, wss = [], []
         with torch.cuda.stream(s):
-            out = torch.empty(48 * max(src.n, 1), dtype=torch.uint8, device=self.dev)
-            ws = torch.empty(max(prep_ws, 256), dtype=torch.uint8, device=self.dev)
-            counts = torch.empty(4, dtype=torch.int64, device=self.dev)
-            _lib.prepare(src.xyz, src.vxyz, src.mHI, src.hsm, src.T, None, src.n, job.obs, out,
-                         counts, ws, s)
-            self.pinned_counts[job.slot].copy_(counts, non_blocking=True)
+            for b, m in enumerate(job.members):
+                if m.obs is None:
+                    m.obs, m.desc, _ = observe(m.spec)
+                m.obs.channel_offset = b * m.spec.n_channels if B > 1 else 0
+                m.obs.nc_total = B * m.spec.n_channels if B > 1 else 0
+                n = m.src.n
+                out = torch.empty(48 * max(n, 1), dtype=torch.uint8, device=self.dev)
+                ws = torch.empty(max(_lib.prepare_workspace_bytes(n), 256), dtype=torch.uint8,
+                                 device=self.dev)
+                _lib.prepare(m.src.xyz, m.src.vxyz, m.src.mHI, m.src.hsm, m.src.T, None, n, m.obs,
+                             out, counts[b], ws, s)
+                outs.append(out)
+                wss.append(ws)
+                m.res.admitted = now
+            self.pinned_counts[job.slot][:B].copy_(counts, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(s)
-        job.bufs = {"out": out, "prep_ws": ws, "counts": counts}
+        d0 = job.members[0].desc
+        job.desc = _lib.make_desc(d0.nx, d0.ny, d0.nc * B, d0.spectrum)
+        for k in range(4):
+            job.desc.gauss_t[k], job.desc.gauss_inv_norm[k] = d0.gauss_t[k], d0.gauss_inv_norm[k]
+        job.bufs = {"outs": outs, "prep_ws": wss, "counts": counts}
         job.event = ev
         job.phase = "prepare"
-        job.res.admitted = now
         return True
 
     def _launch_insert(self, job: _Job, now: float) -> bool:
         torch = self.torch
-        kept, pairs, flags, mask_px = (int(v) for v in self.pinned_counts[job.slot])
-        check_flags(flags, skip_validation=True)
+        B = len(job.members)
+        rows = [[int(v) for v in self.pinned_counts[job.slot][b]] for b in range(B)]
+        for _, _, flags, _ in rows:
+            check_flags(flags, skip_validation=True)
+        kept = sum(r[0] for r in rows)
+        pairs = sum(r[1] for r in rows)
         d = job.desc
         ws_bytes = _lib.workspace_bytes(d, kept, pairs)
         cube_n = d.nx * d.ny * d.nc
-        need = ws_bytes + 4 * cube_n
-        # the setup workspace is no longer needed: its reservation moves to the insertion
-        prep_ws = job.bufs["prep_ws"].numel()
+        need = ws_bytes + 4 * cube_n + (48 * kept if B > 1 else 0)
+        prep_ws = sum(w.numel() for w in job.bufs["prep_ws"])
         if not self._reserve(max(0, need - prep_ws), force=self.in_use - job.reserved == 0):
             self.admission_waits += job.phase != "wait_mem"
             job.phase = "wait_mem"
@@ -281,32 +347,48 @@ class Service:
         s = self.streams[job.slot]
         with torch.cuda.stream(s):
             job.bufs.pop("prep_ws")
+            if B == 1:
+                rec = job.bufs["outs"][0]
+            else:  # the members' records, back to back
+                rec = torch.empty(48 * max(kept, 1), dtype=torch.uint8, device=self.dev)
+                at = 0
+                for out, r in zip(job.bufs["outs"], rows):
+                    if r[0]:
+                        rec[at:at + 48 * r[0]].copy_(out[:48 * r[0]])
+                    at += 48 * r[0]
+                self.fused_launches += 1
             ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=self.dev)
             cube = torch.zeros(cube_n, dtype=torch.float32, device=self.dev)
             if kept > 0:
-                _lib.bin(job.bufs["out"], kept, pairs, d, ws, s)
-                _lib.gather(job.bufs["out"], kept, pairs, d, cube, ws, self.method, s)
+                _lib.bin(rec, kept, pairs, d, ws, s)
+                _lib.gather(rec, kept, pairs, d, cube, ws, self.method, s)
             host = self._pinned_cube(cube_n)
             host.copy_(cube, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(s)
-        job.bufs.update(ws=ws, cube=cube, host=host)
+        job.bufs.update(rec=rec, ws=ws, cube=cube, host=host)
         job.event = ev
         job.phase = "insert"
-        job.res.inserted = now
-        job.res.kept, job.res.pairs = kept, pairs
-        job.res.updates = mask_px * d.nc
+        for m, r in zip(job.members, rows):
+            m.res.inserted = now
+            m.res.kept, m.res.pairs = r[0], r[1]
+            m.res.updates = r[3] * m.spec.n_channels
+            m.res.group = B
         return True
 
     def _finish(self, job: _Job, now: float) -> None:
         host = job.bufs["host"]
         d = job.desc
+        B = len(job.members)
         if self.keep_cubes:
-            job.res.cube = host.numpy().reshape(d.nx, d.ny, d.nc).copy()
+            full = host.numpy().reshape(d.nx, d.ny, B, d.nc // B)
+            for b, m in enumerate(job.members):
+                m.res.cube = full[:, :, b, :].copy()
         self._release_pinned(host)
         job.bufs.clear()
         self.in_use -= job.reserved
-        job.res.done = now
+        for m in job.members:
+            m.res.done = now
         job.phase = "done"
 
     # ---------------------------------------------------------- loop
@@ -317,7 +399,7 @@ class Service:
         torch = self.torch
         torch.cuda.synchronize()
         pending = deque(sorted(zip(specs, sources), key=lambda p: p[0].arrival))
-        waiting: deque[_Job] = deque()
+        waiting: deque[_Member] = deque()
         active: dict[int, _Job] = {}
         results: list[Result] = []
         t0 = time.perf_counter()
@@ -325,7 +407,7 @@ class Service:
             now = time.perf_counter() - t0
             while pending and pending[0][0].arrival <= now:
                 spec, src = pending.popleft()
-                waiting.append(_Job(spec, src, Result(spec.rid, spec.arrival)))
+                waiting.append(_Member(spec, src, Result(spec.rid, spec.arrival)))
             progressed = False
             for slot, job in list(active.items()):
                 if job.phase == "prepare" and job.event.query():
@@ -334,17 +416,17 @@ class Service:
                     progressed |= self._launch_insert(job, now)
                 elif job.phase == "insert" and job.event.query():
                     self._finish(job, time.perf_counter() - t0)
-                    results.append(job.res)
+                    results.extend(m.res for m in job.members)
                     del active[slot]
                     progressed = True
             free = [k for k in range(self.slots) if k not in active]
             while waiting and free:
-                job = waiting[0]
-                job.slot = free[0]
+                group = self._take_group(waiting)
+                job = _Job(group, slot=free[0])
                 if not self._launch_prepare(job, now):
-                    job.slot = -1
+                    for m in reversed(group):
+                        waiting.appendleft(m)
                     break
-                waiting.popleft()
                 active[free.pop(0)] = job
                 progressed = True
             if not progressed:
@@ -372,4 +454,5 @@ def summarize(results: list[Result], wall: float | None = None) -> dict:
         "updates": upd,
         "updates_per_s": upd / wall,
         "particles": int(sum(r.kept for r in results)),
+        "fused_requests": int(sum(r.group > 1 for r in results)),
     }

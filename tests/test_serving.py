@@ -23,18 +23,25 @@ def test_mix_equals_alone():
     mix = serving.Service(slots=4, method="mfma").run(specs, sources)
     summary = serving.summarize(mix)
     print("mix:", {k: round(v, 3) if isinstance(v, float) else v for k, v in summary.items()})
-    alone_service = serving.Service(slots=1, method="mfma")
+    alone_service = serving.Service(slots=1, method="mfma", fuse_max_n=0)
     overlapped = 0
     for r in mix:
         others = [q for q in mix if q.rid != r.rid and q.inserted < r.done and q.done > r.inserted]
         overlapped += bool(others)
+    fused = 0
     for spec, src, r in zip(specs, sources, mix):
         spec0 = serving.RequestSpec(**{**spec.__dict__, "arrival": 0.0})
         (alone,) = alone_service.run([spec0], [src])
-        assert alone.kept == r.kept and alone.pairs == r.pairs
-        assert np.array_equal(alone.cube, r.cube), spec
-    print(f"{overlapped} of {len(mix)} requests overlapped another on the GPU")
-    assert overlapped > 0
+        assert alone.kept == r.kept and alone.group == 1
+        if r.group == 1:
+            assert alone.pairs == r.pairs
+            assert np.array_equal(alone.cube, r.cube), spec
+        else:  # fused: same particles, chunked with other requests' particles
+            fused += 1
+            voxel, flux = parity(r.cube.astype(float), alone.cube.astype(float))
+            assert voxel <= VOXEL_TOL and flux <= FLUX_TOL, (spec, voxel, flux)
+    print(f"{overlapped} of {len(mix)} requests overlapped another on the GPU, {fused} fused")
+    assert overlapped > 0 and fused > 0
 
 
 def test_served_cube_matches_oracle():
