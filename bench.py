@@ -105,6 +105,22 @@ def time_gpu(rec, desc, n_pairs, method: str, steps: int, warmup: int):
     return ms, gather_ms, st, cube
 
 
+def measured_traffic(workload: str, method: str):
+    """HBM bytes per gather-stage launch from the committed FETCH_SIZE /
+    WRITE_SIZE counter runs (profiles/hbm_traffic_gather.md), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_gather.md")
+    if not os.path.exists(path):
+        return None
+    in_stage = False
+    for line in open(path):
+        if line.startswith("Gather stage"):
+            in_stage = True
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if in_stage and len(cells) == 4 and cells[0] == workload and cells[1] == method:
+            return int(cells[2])
+    return None
+
+
 def serving_leg(n_requests: int, method: str, slots: int, load: float, seed: int = 5000) -> dict:
     """The serving form on a request stream: capacity with every request
     waiting at t = 0, then the same requests with Poisson arrivals at `load`
@@ -198,7 +214,7 @@ def main(argv=None) -> int:
     achieved = flops / (gather_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.method]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None}
+                "frac": achieved / peak, "traffic": measured_traffic(args.workload, args.method)}
 
     srv = None
     if args.serving > 0:

@@ -87,7 +87,7 @@ def measure(P: O.Prepared, budget_s: float = 15.0, ncpu: int | None = None) -> d
         "cores": int(ncpu),
         "kind": "port",
         "sample": (f"init_spectra of all {P.h_px.size} particles ({t_spec:.2f} s) + pixel loop "
-                   f"on every {stride}th of {len(all_px)} pixels ({len(sample)} px, {t_loop:.2f} s), "
+                   f"on one pixel in {stride} of {len(all_px)} ({len(sample)} px, {t_loop:.2f} s), "
                    f"loop extrapolated linearly; ThreadPool({ncpu})"),
         "seconds_extrapolated": t_total,
     }
