@@ -164,7 +164,8 @@ __device__ inline float erf_diff(float a, float b) {
 // erf(m + D/2) - erf(m - D/2) for D >= 0.  Narrow spans use the midpoint
 // Taylor series of the integral of exp(-t^2) (terms to D^7; error below 1e-7
 // of the span's peak for D <= 0.5), which has no cancellation and costs one
-// exp; wider spans take erf_diff.
+// hardware exp (v_exp_f32 of -m^2 log2 e: relative error a few 1e-7 for the
+// m^2 <= 25 that carry flux); wider spans take erf_diff.
 __device__ inline float erf_span(float m, float D) {
     if (D <= 0.5f) {
         const float m2 = m * m, D2 = D * D;
@@ -174,7 +175,7 @@ __device__ inline float erf_span(float m, float D) {
         float q = (16.0f * m2 - 48.0f) * m2 + 12.0f;            // H4-type polynomial
         float poly = 1.0f + D2 * ((4.0f * m2 - 2.0f) * (1.0f / 24.0f) +
                                   D2 * (q * (1.0f / 1920.0f) + D2 * p * (1.0f / 322560.0f)));
-        return 1.12837916709551257f * expf(-m2) * D * poly;
+        return 1.12837916709551257f * __expf(-m2) * D * poly;
     }
     return erf_diff(m - 0.5f * D, m + 0.5f * D);
 }
@@ -183,14 +184,14 @@ constexpr double GAUSS_SIG = 0.42466090014400953;  // 1 / (2 sqrt(2 ln 2)): FWHM
 
 // The four closed-form kernels depend on q = dr/h only: w = F(q) / h^2 with
 // F(q) = w(q, 0, 1).  F is tabulated at WTAB_N intervals of [0, 1] from the
-// FP64 closed forms above; entry i holds F at nodes i-1 .. i+2 (F(-q) = F(q))
-// and the weight is the 4-point Lagrange interpolant.  WTAB_N = 500 puts the
-// kernels' seams (cubic q = 1/2, quartic q = 1/5 and 3/5) on nodes; the
-// error, about dq^4 max|F^(4)| / 24, stays below 1e-9 of F(0).  Node WTAB_N
-// holds the left limit F(1-): WendlandC6's rounded coefficients leave it
-// nonzero (~4e-5 of F(0)), so its node beyond 1 is extrapolated from inside;
-// the others vanish smoothly at q = 1.  q >= 1 gives 0, as the reference's
-// strict or zero-valued support edge does.
+// FP64 closed forms above; entry i holds the 4-point Lagrange cubic through
+// the nodes i-1 .. i+2 (F(-q) = F(q)) in monomial form, evaluated by Horner.
+// WTAB_N = 500 puts the kernels' seams (cubic q = 1/2, quartic q = 1/5 and
+// 3/5) on nodes; the error, about dq^4 max|F^(4)| / 24, stays below 1e-9 of
+// F(0).  Node WTAB_N holds the left limit F(1-): WendlandC6's rounded
+// coefficients leave it nonzero (~4e-5 of F(0)), so its node beyond 1 is
+// extrapolated from inside; the others vanish smoothly at q = 1.  q >= 1
+// gives 0, as the reference's strict or zero-valued support edge does.
 constexpr int WTAB_N = 500;
 constexpr int WTAB_KERNELS = 4;  // indexed by kernel kind: cubic, WC2, WC6, quartic
 
@@ -227,15 +228,9 @@ __device__ inline float table_weight(const float4* __restrict__ tab, float q) {
     const float x = q * (float)WTAB_N;
     const int i = min((int)x, WTAB_N - 1);
     const float t = x - (float)i;
-    const float4 f = tab[i];
-    const float tp1 = t + 1.0f, tm1 = t - 1.0f, tm2 = t - 2.0f;
-    const float l0 = -t * tm1 * tm2 * (1.0f / 6.0f);
-    const float l1 = tp1 * tm1 * tm2 * 0.5f;
-    const float l2 = -tp1 * t * tm2 * 0.5f;
-    const float l3 = tp1 * t * tm1 * (1.0f / 6.0f);
-    return l0 * f.x + l1 * f.y + l2 * f.z + l3 * f.w;
+    const float4 c = tab[i];  // the interval's cubic, c.x + c.y t + c.z t^2 + c.w t^3
+    return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
 }
-
 // Per-particle factors of the pixel weight, formed once per particle so the
 // per-pixel evaluation has no division:
 //   table kernels: a = 1/h^2 (FP64, for q^2), b = (float)(1/h^2);
@@ -307,10 +302,13 @@ __device__ inline void fill_wtab(float4* __restrict__ tab) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= WTAB_KERNELS * WTAB_N) return;
     const int which = e / WTAB_N, i = e - which * WTAB_N;
-    tab[e] = make_float4((float)wtab_node(which, i - 1), (float)wtab_node(which, i),
-                         (float)wtab_node(which, i + 1), (float)wtab_node(which, i + 2));
+    // Lagrange cubic through nodes i-1 .. i+2 (t = -1, 0, 1, 2) in monomial form
+    const double fm = wtab_node(which, i - 1), f0 = wtab_node(which, i);
+    const double f1 = wtab_node(which, i + 1), f2 = wtab_node(which, i + 2);
+    tab[e] = make_float4((float)f0, (float)(-fm / 3.0 - f0 / 2.0 + f1 - f2 / 6.0),
+                         (float)(fm / 2.0 - f0 + f1 / 2.0),
+                         (float)(-fm / 6.0 + f0 / 2.0 - f1 / 2.0 + f2 / 6.0));
 }
-
 // Spectrum of a particle in channel c (Jy per unit weight): amp times the
 // Gaussian line integrated over the channel (spectral_models.py:347-425), or
 // amp inside the Dirac window (:510-570).  [lo, hi] is the particle's channel

@@ -55,15 +55,25 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) acc[jj][cc] = 0.0f;
 
-    mi_particle next;  // wave 0: the record this lane stages next
-    if (wave == 0 && t < min(KC, it.end - it.begin)) next = P[order[it.begin + t]];
+    // Chunk j of the item is staged by wave j % 4, so the staging work is spread
+    // over the SIMDs.  Each wave prefetches the record of the next chunk it
+    // stages (j + 4) and the sort index of the one after (j + 8): every global
+    // load has four chunks' time to land.
+    mi_particle next;
+    uint32_t next_idx = 0;
+    {
+        const int b0 = it.begin + wave * KC;
+        if (lane < min(KC, it.end - b0)) next = P[order[b0 + lane]];
+        if (lane < min(KC, it.end - b0 - 4 * KC)) next_idx = order[b0 + 4 * KC + lane];
+    }
+    int chunk = 0;
 
-    for (int kb = it.begin; kb < it.end; kb += KC) {
+    for (int kb = it.begin; kb < it.end; kb += KC, ++chunk) {
         const int nk = min(KC, it.end - kb);
-        if (wave == 0) {
-            stage_chunk(next, t < nk, x0, y0, c0, g, d, cs);
-            const int nk2 = min(KC, it.end - kb - KC);
-            if (t < nk2) next = P[order[kb + KC + t]];
+        if (wave == (chunk & 3)) {
+            stage_chunk(next, lane < nk, x0, y0, c0, g, d, cs);
+            if (lane < min(KC, it.end - kb - 4 * KC)) next = P[next_idx];
+            if (lane < min(KC, it.end - kb - 8 * KC)) next_idx = order[kb + 8 * KC + lane];
         }
         __syncthreads();
         // Particles whose footprint meets this wave's 4 rows (wave-uniform).

@@ -3,9 +3,10 @@
 // clipped to the cell's tile, and W evaluated once per (particle, footprint
 // pixel) pair.
 //
-// Staging is done by wave 0 with wave-wide scans (no serial loop), and the
-// next chunk's records are loaded while the current chunk is processed, so
-// the global-memory latency is hidden behind the W/S/accumulate work.
+// Staging is done by one wave with wave-wide scans (no serial loop); the
+// waves take turns, and each loads its next chunk's records while earlier
+// chunks are processed, so the global-memory latency is hidden behind the
+// W/S/accumulate work.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,7 +31,7 @@ struct ChunkState {
     uint32_t chanmask;  // 16-channel sub-blocks of the cell touched by any window
 };
 
-// Called by all 64 lanes of wave 0: lane t < KC holds particle t of the chunk
+// Called by all 64 lanes of one wave: lane t < KC holds particle t of the chunk
 // in `p` (valid when t < nk).  Clips footprints to the tile, scans the pair
 // counts, ORs the row and channel-block masks across the wave and forms each
 // particle's weight factors.
@@ -150,10 +151,11 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
 // Evaluates S for the chunk's KC particles x CBLOCK channels of the cell and
 // hands each value to store(k, channel-in-block, s); particles k >= nk and
 // channels beyond the cube give 0.  Thread t takes particle t % KC, so its
-// record fields are read once.
+// record fields are read once.  Only the 16-channel blocks set in `blocks`
+// (wave-uniform) are evaluated; the others are left untouched.
 template <class Store>
 __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc, int spectrum,
-                                    Store store) {
+                                    Store store, uint32_t blocks = 0xFu) {
     const int k = threadIdx.x & (KC - 1);
     const int cl0 = threadIdx.x / KC;
     const mi_particle& p = cs.p[k];
@@ -163,6 +165,7 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     const float fu = p.fu, s = p.s, amp = p.amp;
 #pragma unroll
     for (int cl = cl0; cl < CBLOCK; cl += NT / KC) {
+        if (!((blocks >> (cl >> 4)) & 1u)) continue;  // uniform: a wave's lanes share cl >> 4
         const int c = c0 + cl;
         store(k, cl, c < nc ? line_flux(lo, hi, k0, fu, s, amp, c, spectrum) : 0.0f);
     }
