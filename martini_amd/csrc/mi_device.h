@@ -186,13 +186,13 @@ constexpr double GAUSS_SIG = 0.42466090014400953;  // 1 / (2 sqrt(2 ln 2)): FWHM
 // F(q) = w(q, 0, 1).  F is tabulated at WTAB_N intervals of [0, 1] from the
 // FP64 closed forms above; entry i holds the 4-point Lagrange cubic through
 // the nodes i-1 .. i+2 (F(-q) = F(q)) in monomial form, evaluated by Horner.
-// WTAB_N = 500 puts the kernels' seams (cubic q = 1/2, quartic q = 1/5 and
-// 3/5) on nodes; the error, about dq^4 max|F^(4)| / 24, stays below 1e-9 of
-// F(0).  Node WTAB_N holds the left limit F(1-): WendlandC6's rounded
+// WTAB_N = 250 puts the kernels' seams (cubic q = 1/2, quartic q = 1/5 and
+// 3/5) on nodes; the error, about dq^4 max|F^(4)| / 24, stays below 2e-8 of
+// F(0), and one kernel's table (4 KB) fits in LDS beside the tiles.  Node WTAB_N holds the left limit F(1-): WendlandC6's rounded
 // coefficients leave it nonzero (~4e-5 of F(0)), so its node beyond 1 is
 // extrapolated from inside; the others vanish smoothly at q = 1.  q >= 1
 // gives 0, as the reference's strict or zero-valued support edge does.
-constexpr int WTAB_N = 500;
+constexpr int WTAB_N = 250;
 constexpr int WTAB_KERNELS = 4;  // indexed by kernel kind: cubic, WC2, WC6, quartic
 
 // The reference's cubic I3 exceeds the exact outer-shell integral by R2*A/8
@@ -267,13 +267,17 @@ __device__ inline WFactors weight_factors(int kind, float h, float gt, float gin
 // u = dr/(h sig), are formed in FP64: zmax has a square-root singularity at
 // the truncation radius, where the t = 3 kernel is still 1% of its peak, so
 // FP32 rounding of u would show there at the 1e-5 level.
+// `local`, when given, is a copy of kernel `local_kind`'s table (in LDS), read
+// instead of wtab for that kind.
 __device__ inline float weight_at(int kind, double dx, double dy, const WFactors& f,
-                                  const float4* __restrict__ wtab) {
+                                  const float4* __restrict__ wtab, int local_kind = -1,
+                                  const float4* __restrict__ local = nullptr) {
     const double dr2 = dx * dx + dy * dy;
     if (kind <= MI_KERNEL_QUARTIC) {
         const float q2 = (float)(dr2 * f.a);
         const float q = sqrtf(q2);
-        float w = table_weight(wtab + kind * WTAB_N, q);
+        float w = kind == local_kind ? table_weight(local, q)
+                                     : table_weight(wtab + kind * WTAB_N, q);
         if (kind == MI_KERNEL_CUBIC && q2 <= 0.25f) {
             const float R2 = 4.0f * q2;
             w += R2 * sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));

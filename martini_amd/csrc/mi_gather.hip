@@ -23,7 +23,8 @@
 
 namespace mi {
 
-__global__ __launch_bounds__(NT) void k_gather_plain(
+// Three waves per SIMD (at most 168 VGPRs): three work groups share a CU.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_gather_plain(
     const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
     const Item* __restrict__ items,
     const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
@@ -38,6 +39,7 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
     __shared__ __attribute__((aligned(16))) float Wt[KC][TILE_PX];
     __shared__ __attribute__((aligned(16))) float St[KC][CBLOCK];
     __shared__ ChunkState cs;
+    __shared__ float4 Tk[WTAB_N];  // the item's kernel table
 
     const int t = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
@@ -59,6 +61,7 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
     // over the SIMDs.  Each wave prefetches the record of the next chunk it
     // stages (j + 4) and the sort index of the one after (j + 8): every global
     // load has four chunks' time to land.
+    const int tkind = stage_table(P, order, it.begin, wtab, Tk);
     mi_particle next;
     uint32_t next_idx = 0;
     {
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(NT) void k_gather_plain(
         const int klo = cs.ilo[lane & (KC - 1)], khi = cs.ihi[lane & (KC - 1)];
         const bool hit = lane < KC && klo <= khi && khi >= wrow0 && klo <= wrow1;
         uint32_t kmask = (uint32_t)__ballot(hit);
-        fill_weights(cs, x0, y0, wtab, [&](int k, int px, float w) { Wt[k][px] = w; });
+        fill_weights(cs, x0, y0, wtab, tkind, Tk, [&](int k, int px, float w) { Wt[k][px] = w; });
         fill_spectra(cs, nk, c0, g.nc, d.spectrum, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
         if (kmask == 0u) continue;

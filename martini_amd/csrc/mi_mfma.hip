@@ -75,6 +75,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ __attribute__((aligned(16))) char Bh[CBLOCK * KROW];   // S hi [channel][k]
     __shared__ __attribute__((aligned(16))) char Bl[CBLOCK * KROW];   // S lo
     __shared__ ChunkState cs;
+    __shared__ float4 Tk[WTAB_N];  // the item's kernel table
 
     const int t = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
@@ -97,6 +98,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // over the SIMDs.  Each wave prefetches the record of the next chunk it
     // stages (j + 4) and the sort index of the one after (j + 8): every global
     // load has four chunks' time to land.
+    const int tkind = stage_table(P, order, it.begin, wtab, Tk);
     mi_particle next;
     uint32_t next_idx = 0;
     {
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         __syncthreads();
         const uint32_t rows = (__builtin_amdgcn_readfirstlane(cs.rowmask) >> (wave * 4)) & 15u;
         const uint32_t chans = __builtin_amdgcn_readfirstlane(cs.chanmask);
-        fill_weights(cs, x0, y0, wtab,
+        fill_weights(cs, x0, y0, wtab, tkind, Tk,
                      [&](int k, int px, float w) { put_split(Ah, Al, px, k, w); });
         fill_spectra(cs, nk, c0, g.nc, d.spectrum,
                      [&](int k, int cl, float s) { put_split(Bh, Bl, cl, k, s); }, chans);

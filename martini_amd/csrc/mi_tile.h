@@ -104,7 +104,8 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
 // the particle's values in registers until its pairs run out.
 template <class Store>
 __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
-                                    const float4* __restrict__ wtab, Store store) {
+                                    const float4* __restrict__ wtab, int tab_kind,
+                                    const float4* __restrict__ tab, Store store) {
     const int total = cs.off[KC];
     const int per = (total + NT - 1) / NT;
     int q = threadIdx.x * per;
@@ -140,7 +141,8 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
             f = WFactors{cs.fa[k], cs.fe[k], cs.fb[k], cs.ft[k], cs.fn[k]};
         }
         const int i = ilo + a, j = jlo + b;
-        store(k, (i - x0) * TILE + (j - y0), weight_at(kind, px - (double)i, py - (double)j, f, wtab));
+        store(k, (i - x0) * TILE + (j - y0),
+              weight_at(kind, px - (double)i, py - (double)j, f, wtab, tab_kind, tab));
         if (++b == width) {
             b = 0;
             ++a;
@@ -169,6 +171,20 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
         const int c = c0 + cl;
         store(k, cl, c < nc ? line_flux(lo, hi, k0, fu, s, amp, c, spectrum) : 0.0f);
     }
+}
+
+// Copies the table of the kernel kind of the item's first particle (when it
+// is a tabulated kind) into LDS; returns that kind, or -1.  Particles of one
+// cell are sorted with the tabulated kinds first, so this is the table the
+// item uses (a second tabulated kind in the same cell reads the global copy).
+// The copy is complete after the next __syncthreads().
+__device__ inline int stage_table(const mi_particle* __restrict__ P,
+                                  const uint32_t* __restrict__ order, int begin,
+                                  const float4* __restrict__ wtab, float4* local) {
+    const int kind = kind_of(P[order[begin]].r_kind);
+    if (kind > MI_KERNEL_QUARTIC) return -1;
+    for (int e = threadIdx.x; e < WTAB_N; e += NT) local[e] = wtab[kind * WTAB_N + e];
+    return kind;
 }
 
 // Cell coordinates of a work item.
