@@ -275,19 +275,20 @@ __device__ inline float weight_at(int kind, double dx, double dy, const WFactors
     const double dr2 = dx * dx + dy * dy;
     if (kind <= MI_KERNEL_QUARTIC) {
         const float q2 = (float)(dr2 * f.a);
-        const float q = sqrtf(q2);
+        const float q = __builtin_amdgcn_sqrtf(q2);  // hardware sqrt, within 1 ulp
         float w = kind == local_kind ? table_weight(local, q)
                                      : table_weight(wtab + kind * WTAB_N, q);
         if (kind == MI_KERNEL_CUBIC && q2 <= 0.25f) {
             const float R2 = 4.0f * q2;
-            w += R2 * sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
+            w += R2 * __builtin_amdgcn_sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
         }
         return w * f.b;
     }
     if (kind == MI_KERNEL_GAUSS) {
         if (dr2 > f.e) return 0.0f;
         const double z2 = (double)f.t * (double)f.t - dr2 * f.a;
-        const float zmax = z2 > 0.0 ? (float)sqrt(z2) : 0.0f;
+        // z2 needs FP64 (t^2 - u^2 cancels); its root does not
+        const float zmax = z2 > 0.0 ? __builtin_amdgcn_sqrtf((float)z2) : 0.0f;
         const float ex = erf_span((float)dx * f.b, f.b);
         const float ey = erf_span((float)dy * f.b, f.b);
         return 0.25f * erff(zmax * 0.70710678118654752f) * ex * ey * f.n;

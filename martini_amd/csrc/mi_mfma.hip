@@ -124,12 +124,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                      [&](int k, int cl, float s) { put_split(Bh, Bl, cl, k, s); }, chans);
         __syncthreads();
         if (rows != 0u) {
-            bf16x8 bh[4], bl[4];
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                bh[nb] = get8(Bh, nb * 16 + m, kg);
-                bl[nb] = get8(Bl, nb * 16 + m, kg);
-            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 if (!(rows & (1u << r))) continue;
@@ -138,10 +132,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
                 for (int nb = 0; nb < 4; ++nb) {
                     if (!(chans & (1u << nb))) continue;
+                    const bf16x8 bh = get8(Bh, nb * 16 + m, kg), bl = get8(Bl, nb * 16 + m, kg);
                     floatx4 c = acc[r][nb];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[nb], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[nb], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[nb], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
                     acc[r][nb] = c;
                 }
                 // Clear the row's 1 KiB in both W tiles (one 16-byte store per lane each);

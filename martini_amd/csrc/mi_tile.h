@@ -19,14 +19,23 @@ namespace mi {
 constexpr int KC = 32;    // particles per chunk
 constexpr int NT = 256;   // threads per work group
 
+// What the W walk needs of a particle, read with two 16-byte loads whenever
+// a thread moves on to the next particle.
+struct __attribute__((aligned(16))) WalkRec {
+    double px, py;  // particle position ix + fx, iy + fy (exact in FP64)
+    double fa;      // WFactors::a: 1/h^2, or 1/(h sig)^2 for the Gaussian
+    int32_t ij;     // first row | first column << 16 of the clipped footprint
+    int32_t kw;     // kernel kind | footprint width << 8
+};
+
 struct ChunkState {
     mi_particle p[KC];
-    int ilo[KC], ihi[KC], jlo[KC], jhi[KC];
+    WalkRec walk[KC];
+    double ge[KC];      // Gaussian kernel: WFactors::e
+    float4 gf[KC];      // Gaussian kernel: WFactors::b, t, n
+    int ilo[KC], ihi[KC];
     int wdiv[KC];       // ceil(2^16 / footprint width): (n * wdiv) >> 16 == n / width for n < 4096
     int off[KC + 1];    // exclusive scan of the per-particle pair counts
-    double px[KC], py[KC];   // particle position ix + fx, iy + fy (exact in FP64)
-    double fa[KC], fe[KC];   // WFactors of each particle (mi_device.h)
-    float fb[KC], ft[KC], fn[KC];
     uint32_t rowmask;   // tile rows (x offsets) touched by any footprint of the chunk
     uint32_t chanmask;  // 16-channel sub-blocks of the cell touched by any window
 };
@@ -71,21 +80,23 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
         if (valid) cs.p[lane] = p;
         cs.ilo[lane] = ilo;
         cs.ihi[lane] = ihi;
-        cs.jlo[lane] = jlo;
-        cs.jhi[lane] = jhi;
         const int width = max(jhi - jlo + 1, 1);
         cs.wdiv[lane] = (65536 + width - 1) / width;
         cs.off[lane + 1] = incl;
         if (cnt > 0) {
-            cs.px[lane] = (double)p.ix + (double)p.fx;
-            cs.py[lane] = (double)p.iy + (double)p.fy;
             const int kind = kind_of(p.r_kind), gi = gidx_of(p.r_kind);
             const WFactors f = weight_factors(kind, p.h, d.gauss_t[gi], d.gauss_inv_norm[gi]);
-            cs.fa[lane] = f.a;
-            cs.fe[lane] = f.e;
-            cs.fb[lane] = f.b;
-            cs.ft[lane] = f.t;
-            cs.fn[lane] = f.n;
+            WalkRec w;
+            w.px = (double)p.ix + (double)p.fx;
+            w.py = (double)p.iy + (double)p.fy;
+            w.fa = f.a;
+            w.ij = ilo | (jlo << 16);
+            w.kw = kind | (width << 8);
+            cs.walk[lane] = w;
+            if (kind == MI_KERNEL_GAUSS) {
+                cs.ge[lane] = f.e;
+                cs.gf[lane] = make_float4(f.b, f.t, f.n, 0.0f);
+            }
         }
     }
     if (lane == 0) {
@@ -116,33 +127,34 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
     for (int step = KC / 2; step > 0; step >>= 1)
         if (cs.off[k + step] <= q) k += step;
     int kend = cs.off[k + 1];
-    int width = cs.jhi[k] - cs.jlo[k] + 1;
+    WalkRec w = cs.walk[k];
+    int width = w.kw >> 8;
     const int local = q - cs.off[k];
     int a = (local * cs.wdiv[k]) >> 16;
     int b = local - a * width;
-    int ilo = cs.ilo[k], jlo = cs.jlo[k];
-    double px = cs.px[k], py = cs.py[k];
-    int kind = kind_of(cs.p[k].r_kind);
-    WFactors f{cs.fa[k], cs.fe[k], cs.fb[k], cs.ft[k], cs.fn[k]};
     for (; q < qend; ++q) {
         if (q >= kend) {  // next particle that has pairs
             do {
                 ++k;
                 kend = cs.off[k + 1];
             } while (q >= kend);
-            width = cs.jhi[k] - cs.jlo[k] + 1;
+            w = cs.walk[k];
+            width = w.kw >> 8;
             a = 0;
             b = 0;
-            ilo = cs.ilo[k];
-            jlo = cs.jlo[k];
-            px = cs.px[k];
-            py = cs.py[k];
-            kind = kind_of(cs.p[k].r_kind);
-            f = WFactors{cs.fa[k], cs.fe[k], cs.fb[k], cs.ft[k], cs.fn[k]};
         }
-        const int i = ilo + a, j = jlo + b;
+        const int kind = w.kw & 255;
+        const int i = (w.ij & 0xffff) + a, j = (w.ij >> 16) + b;
+        WFactors f{w.fa, 0.0, (float)w.fa, 0.0f, 0.0f};
+        if (kind == MI_KERNEL_GAUSS) {
+            const float4 g = cs.gf[k];
+            f.e = cs.ge[k];
+            f.b = g.x;
+            f.t = g.y;
+            f.n = g.z;
+        }
         store(k, (i - x0) * TILE + (j - y0),
-              weight_at(kind, px - (double)i, py - (double)j, f, wtab, tab_kind, tab));
+              weight_at(kind, w.px - (double)i, w.py - (double)j, f, wtab, tab_kind, tab));
         if (++b == width) {
             b = 0;
             ++a;
