@@ -231,6 +231,19 @@ int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
                const mi_observation* obs, mi_particle* d_out, uint64_t* d_counts,
                void* d_workspace, size_t workspace_bytes, void* stream);
 
+/*
+ * Beam convolution of a cube: every channel map of d_in ([nx][ny][nc],
+ * FP32) convolved with the kx x ky beam image d_kernel (row-major, both odd),
+ * 'same' output size with zero boundary, written to d_out (not aliasing
+ * d_in).  Replaces the per-channel scipy.signal.fftconvolve(..., mode="same")
+ * of Martini.convolve_beam (martini/martini.py:863-901); the caller pads the
+ * cube by (kx/2, ky/2) first (DataCube.add_pad, martini/datacube.py:669-708)
+ * and drops the pad after.  Asynchronous.  MI_ERR_LIMIT for beam images
+ * wider than about 250 pixels.
+ */
+int mi_convolve_beam(const float* d_in, float* d_out, int32_t nx, int32_t ny, int32_t nc,
+                     const float* d_kernel, int32_t kx, int32_t ky, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

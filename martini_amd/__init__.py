@@ -11,8 +11,8 @@ from .datacube import DataCube, HIfreq
 from .L_coords import L_coords
 from .martini import Martini
 from .sources import SPHSource
-from . import sph_kernels, spectral_models
+from . import beams, sph_kernels, spectral_models
 
-__all__ = ["Martini", "DataCube", "SPHSource", "L_coords", "HIfreq", "sph_kernels",
+__all__ = ["Martini", "DataCube", "SPHSource", "L_coords", "HIfreq", "beams", "sph_kernels",
            "spectral_models"]
 __version__ = "0.1.0"

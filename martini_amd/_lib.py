@@ -112,6 +112,9 @@ def lib() -> ctypes.CDLL:
     L.mi_kernel_weights.argtypes = [ctypes.c_int, f32, f32, vp, vp, vp, vp, i64, vp]
     L.mi_spectra.restype = ctypes.c_int
     L.mi_spectra.argtypes = [vp, i64, ctypes.POINTER(CubeDesc), vp, vp]
+    L.mi_convolve_beam.restype = ctypes.c_int
+    L.mi_convolve_beam.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp,
+                                   ctypes.c_int32, ctypes.c_int32, vp]
     L.mi_prepare_workspace_size.restype = ctypes.c_int
     L.mi_prepare_workspace_size.argtypes = [i64, ctypes.POINTER(ctypes.c_size_t)]
     L.mi_prepare.restype = ctypes.c_int
@@ -252,3 +255,11 @@ def prepare(d_xyz, d_vxyz, d_mHI, d_hsm, d_T, d_edges, n: int, obs: Observation,
                            _ptr(d_edges), int(n), ctypes.byref(obs), _ptr(d_out), _ptr(d_counts),
                            _ptr(d_ws), d_ws.numel() * d_ws.element_size(), stream_ptr(stream)),
           "mi_prepare")
+
+
+def convolve_beam(d_in, d_out, nx: int, ny: int, nc: int, d_kernel, kx: int, ky: int,
+                  stream=None) -> None:
+    """Asynchronous beam convolution of a device cube (mi_convolve_beam)."""
+    check(lib().mi_convolve_beam(_ptr(d_in), _ptr(d_out), int(nx), int(ny), int(nc),
+                                 _ptr(d_kernel), int(kx), int(ky), stream_ptr(stream)),
+          "mi_convolve_beam")

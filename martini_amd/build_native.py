@@ -25,7 +25,8 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["mi_bin.hip", "mi_gather.hip", "mi_mfma.hip", "mi_prepare.hip", "mi_capi.hip"]
+SOURCES = ["mi_bin.hip", "mi_gather.hip", "mi_mfma.hip", "mi_prepare.hip", "mi_beam.hip",
+           "mi_capi.hip"]
 
 
 def _compile(src: str) -> str:

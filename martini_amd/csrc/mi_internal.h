@@ -89,6 +89,11 @@ hipError_t prepare_particles(const double* xyz, const double* vxyz, const double
                              unsigned long long* counts, char* ws, const PrepLayout& L,
                              hipStream_t s);
 
+// Beam convolution of a cube (mi_beam.hip).
+size_t beam_lds_bytes(int kx, int ky);
+hipError_t convolve_beam(const float* in, float* out, int nx, int ny, int nc, const float* K,
+                         int kx, int ky, hipStream_t s);
+
 // One work item: particles [begin, end) of the sorted pair list belonging to
 // one cell; slot >= 0 names its partial tile, -1 means it writes the cube.
 struct Item {

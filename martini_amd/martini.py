@@ -11,32 +11,37 @@ The pixel loop of the reference (:243-283, :327-362) is replaced by the GPU
 path: records packed on the host (``_pack``), then binning, a tile gather
 (``method='plain'`` FP32 or ``'mfma'`` BF16x3) and a partial-tile reduction
 on the device.  ``ncpu`` and ``progressbar`` are accepted for compatibility
-and have no effect.  Beam, noise and output writers are not part of this
-package: passing a beam or noise raises NotImplementedError.
+and have no effect.  A beam pads the cube at construction and
+``convolve_beam`` convolves it on the GPU (martini.py:138-140, :863-901).
+Noise and the output writers are not part of this package: passing noise
+raises NotImplementedError.
 """
 
 from __future__ import annotations
 
+import warnings
+
 import numpy as np
 
-from . import _pack
+from . import _lib, _pack
 from .insertion import default_inserter
 
 
 class _BaseMartini:
     def __init__(self, *, source, datacube, beam=None, noise=None, sph_kernel, spectral_model,
                  quiet: bool = False, _prune_kwargs: dict | None = None) -> None:
-        if beam is not None:
-            raise NotImplementedError("beam convolution is not part of martini_amd")
         if noise is not None:
             raise NotImplementedError("noise is not part of martini_amd")
         self.quiet = quiet
         self.source = source
         self._datacube = datacube
-        self.beam = None
+        self.beam = beam
         self.noise = None
         self.sph_kernel = sph_kernel
         self.spectral_model = spectral_model
+        if self.beam is not None:
+            self.beam.init_kernel(self._datacube)
+            self._datacube.add_pad(self.beam.needs_pad())
         self.source._init_skycoords()
         self.source._init_pixcoords(self._datacube)
         self.sph_kernel._init_sm_lengths(source=self.source, datacube=self._datacube)
@@ -121,10 +126,13 @@ class _BaseMartini:
               sep="\n")
 
     def reset(self) -> None:
-        """Zero the cube, keeping its geometry and padding (martini.py:409-425)."""
+        """A zero cube of the same geometry, padded for the beam (martini.py:409-425)."""
         dc = self._datacube
+        dc.padx = dc.pady = 0
         dc._array = np.zeros(dc._shape())
         dc._array_unit = "Jy/pix2"
+        if self.beam is not None:
+            dc.add_pad(self.beam.needs_pad())
 
 
 class Martini(_BaseMartini):
@@ -138,6 +146,44 @@ class Martini(_BaseMartini):
     @property
     def datacube(self):
         return self._datacube
+
+    def convolve_beam(self) -> None:
+        """Convolve every channel map with the beam image, drop the padding and
+        convert Jy/arcsec^2 to Jy/beam (martini.py:863-901); on the GPU."""
+        from .insertion import default_inserter
+
+        if self.beam is None:
+            warnings.warn("Skipping beam convolution, no beam object provided to Martini.")
+            return
+        dc = self._datacube
+        pad = self.beam.needs_pad()
+        if dc.padx < pad[0] or dc.pady < pad[1]:
+            raise ValueError("datacube padding insufficient for beam convolution (perhaps you "
+                             "loaded a datacube state with datacube.load_state that was "
+                             "previously initialized by martini with a smaller beam?)")
+        ins = default_inserter()
+        torch = ins.torch
+        arr = dc._array[..., 0] if dc.stokes_axis else dc._array
+        nx, ny, nc = arr.shape
+        d_in = torch.from_numpy(np.ascontiguousarray(arr, np.float32)).to(ins.device)
+        d_out = torch.empty_like(d_in)
+        K = np.ascontiguousarray(self.beam.kernel, np.float32)
+        d_k = torch.from_numpy(K).to(ins.device)
+        _lib.convolve_beam(d_in, d_out, nx, ny, nc, d_k, K.shape[0], K.shape[1])
+        out = d_out.cpu().numpy().astype(np.float64)
+        dc._array = out[..., np.newaxis] if dc.stokes_axis else out
+        dc.drop_pad()
+        if dc._array_unit == "Jy/pix2":
+            dc._array = dc._array / dc.px_size**2
+        dc._array = dc._array * self.beam.area  # Jy/arcsec^2 -> Jy/beam
+        dc._array_unit = "Jy/beam"
+        if not self.quiet:
+            a = dc._array
+            print("Beam convolved.",
+                  f"  Data cube RMS after beam convolution: {np.std(a):.2e} Jy / beam",
+                  f"  Maximum pixel: {a.max():.2e} Jy / beam",
+                  f"  Median non-zero pixel: {np.median(a[a > 0]) if (a > 0).any() else 0.0:.2e}"
+                  " Jy / beam", sep="\n")
 
     def insert_source_in_cube(self, skip_validation: bool = False, progressbar=None,
                               ncpu: int = 1, method: str = "plain") -> None:

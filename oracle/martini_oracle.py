@@ -663,3 +663,21 @@ def inserted_mass(cube_jy_arcsec2, cube: CubeSpec, distance_mpc):
     F = (cube_jy_arcsec2 * cube.px_size_arcsec**2).sum((0, 1))
     dv = np.abs(np.diff(cube.velocity_channel_edges())) / 1000.0
     return np.sum(MHI_TO_JY * distance_mpc**2 * F * dv)
+
+
+# ------------------------------------------------------------ beam convolution
+def convolve_beam(cube, kernel, padx, pady):
+    """Martini.convolve_beam (martini/martini.py:863-901): every channel map of
+    the padded cube convolved with the beam image by
+    scipy.signal.fftconvolve(..., mode="same"), then the padding dropped
+    (DataCube.drop_pad, datacube.py:710-729).  The caller multiplies by the
+    beam area to go from Jy/arcsec^2 to Jy/beam.  Pinned by the convolution
+    theorem (scipy's FFT) and, in tests, by a direct sum on small maps."""
+    from scipy.signal import fftconvolve
+
+    out = np.empty_like(cube, dtype=float)
+    for c in range(cube.shape[2]):
+        out[:, :, c] = fftconvolve(cube[:, :, c], kernel, mode="same")
+    if padx > 0 and pady > 0:
+        out = out[padx:-padx, pady:-pady, :]
+    return out

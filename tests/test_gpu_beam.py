@@ -1,0 +1,68 @@
+"""Beam convolution on the GPU (mi_convolve_beam) against scipy's FFT
+convolution, alone and inside Martini.convolve_beam."""
+
+import numpy as np
+import pytest
+from scipy.signal import fftconvolve
+
+from martini_amd import Martini, _lib, synthetic
+from martini_amd.beams import GaussianBeam
+from oracle import cube_c
+from oracle import martini_oracle as O
+from tests.helpers import FLUX_TOL, VOXEL_TOL, oracle_kernel, parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _kernel(bmaj_px, bmin_px, bpa):
+    from martini_amd import DataCube
+
+    b = GaussianBeam(bmaj=bmaj_px, bmin=bmin_px, bpa=bpa, truncate=4)
+    b.init_kernel(DataCube(n_px_x=1, n_px_y=1, n_channels=1, px_size=1.0, channel_width=1.0))
+    return b.kernel
+
+
+@pytest.mark.parametrize("shape,beam", [
+    ((70, 61, 13), (5.0, 3.0, 30.0)),   # channels not a multiple of 4, elliptical, rotated
+    ((48, 48, 8), (14.0, 14.0, 0.0)),   # a 115 px beam image: 17 LDS bands, wider than the map
+])
+def test_convolution_matches_fftconvolve(shape, beam):
+    import torch
+
+    rng = np.random.default_rng(7)
+    cube = rng.random(shape) * (rng.random(shape) > 0.7)
+    K = _kernel(*beam)
+    d_in = torch.from_numpy(cube.astype(np.float32)).cuda()
+    d_out = torch.empty_like(d_in)
+    d_k = torch.from_numpy(K.astype(np.float32)).cuda()
+    _lib.convolve_beam(d_in, d_out, *shape, d_k, K.shape[0], K.shape[1])
+    gpu = d_out.cpu().numpy().astype(float)
+    ref = np.stack([fftconvolve(cube[:, :, c], K, mode="same") for c in range(shape[2])], axis=2)
+    err = np.abs(gpu - ref).max() / np.abs(ref).max()
+    print(f"{shape} beam {beam} kernel {K.shape}: max err {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_convolve_beam_pipeline_matches_oracle():
+    """Insertion into the padded cube, beam convolution, unpadding and
+    Jy/beam conversion against the FP64 oracle with scipy's FFT convolution."""
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    n = 4000
+    beam = GaussianBeam(bmaj=40.0, bmin=25.0, bpa=35.0)
+    src, dc, k, s = synthetic.build(w, n)
+    m = Martini(source=src, datacube=dc, beam=beam, sph_kernel=k, spectral_model=s, quiet=True)
+    m.insert_source_in_cube(method="plain")
+    m.convolve_beam()
+    gpu = m.datacube._array
+    padx, pady = beam.needs_pad()
+    a = w.arrays(n)
+    cube = O.CubeSpec(w.n_px, w.n_px, w.n_channels, w.px_size, w.channel_width,
+                      spectral_centre=w.vsys, padx=padx, pady=pady)
+    P = O.prepare(a.xyz, a.vxyz, a.mHI, a.hsm, a.T, distance_mpc=w.distance, cube=cube,
+                  kernel=oracle_kernel(w.kernel), rotation=w.rotation.as_matrix())
+    ref = O.convolve_beam(cube_c.cube(P) / w.px_size**2, beam.kernel, padx, pady) * beam.area
+    assert gpu.shape == ref.shape == (w.n_px, w.n_px, w.n_channels)
+    voxel, flux = parity(gpu, ref)
+    print(f"beam pipeline: voxel {voxel:.3e} flux {flux:.3e} (beam image {beam.kernel.shape})")
+    assert voxel <= VOXEL_TOL and flux <= FLUX_TOL
+    assert m.datacube._array_unit == "Jy/beam" and (m.datacube.padx, m.datacube.pady) == (0, 0)

@@ -213,7 +213,7 @@ def test_reference_exceptions():
     with pytest.raises(RuntimeError, match="already padded"):
         dc.add_pad((1, 1))
     with pytest.raises(NotImplementedError):
-        Martini(source=src, datacube=dc, beam=object(), sph_kernel=K.CubicSplineKernel(),
+        Martini(source=src, datacube=dc, noise=object(), sph_kernel=K.CubicSplineKernel(),
                 spectral_model=GaussianSpectrum())
     with pytest.raises(NotImplementedError):
         K.AdaptiveKernel()
