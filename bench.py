@@ -121,6 +121,41 @@ def measured_traffic(workload: str, method: str):
     return None
 
 
+def beam_leg(desc, fwhm_px: float, reps: int = 5) -> dict:
+    """Time mi_convolve_beam on a cube of the workload's shape with a circular
+    Gaussian beam of fwhm_px pixels (truncated at 4 FWHM, the reference's
+    default); roofline on 2 flops per tap and voxel against FP32 peak."""
+    import torch
+
+    from martini_amd import DataCube, _lib
+    from martini_amd.beams import GaussianBeam
+
+    b = GaussianBeam(bmaj=fwhm_px, bmin=fwhm_px, truncate=4)
+    b.init_kernel(DataCube(n_px_x=1, n_px_y=1, n_channels=1, px_size=1.0, channel_width=1.0))
+    K = np.ascontiguousarray(b.kernel, np.float32)
+    px, py = b.needs_pad()
+    nx, ny, nc = desc.nx + 2 * px, desc.ny + 2 * py, desc.nc
+    dev = torch.device("cuda", 0)
+    d_in = torch.rand((nx, ny, nc), dtype=torch.float32, device=dev)
+    d_out = torch.empty_like(d_in)
+    d_k = torch.from_numpy(K).to(dev)
+    stream = torch.cuda.current_stream()
+    _lib.convolve_beam(d_in, d_out, nx, ny, nc, d_k, K.shape[0], K.shape[1], stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        _lib.convolve_beam(d_in, d_out, nx, ny, nc, d_k, K.shape[0], K.shape[1], stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flops = 2.0 * nx * ny * nc * K.shape[0] * K.shape[1]
+    return {"cube": f"{nx}x{ny}x{nc} (padded)", "beam_fwhm_px": fwhm_px,
+            "beam_image_px": int(K.shape[0]), "ms": ms,
+            "achieved_tflops": flops / (ms * 1e-3) / 1e12,
+            "frac_fp32_peak": flops / (ms * 1e-3) / 1e12 / PEAK_TFLOPS["plain"]}
+
+
 def serving_leg(n_requests: int, method: str, slots: int, load: float, seed: int = 5000) -> dict:
     """The serving form on a request stream: capacity with every request
     waiting at t = 0, then the same requests with Poisson arrivals at `load`
@@ -181,6 +216,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="one device copy of this many MiB before timing (counter calibration)")
+    ap.add_argument("--beam-fwhm-px", type=float, default=3.0,
+                    help="beam FWHM (pixels) of the beam-convolution timing (0 skips it)")
     ap.add_argument("--serving", type=int, default=512,
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serving-method", default="mfma", choices=["plain", "mfma"])
@@ -215,6 +252,11 @@ def main(argv=None) -> int:
     peak = PEAK_TFLOPS[args.method]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": measured_traffic(args.workload, args.method)}
+
+    beam = beam_leg(desc, args.beam_fwhm_px) if args.beam_fwhm_px > 0 else None
+    if beam is not None:
+        log(f"[bench] beam convolution {beam['cube']} with a {beam['beam_image_px']} px beam "
+            f"image: {beam['ms']:.3f} ms")
 
     srv = None
     if args.serving > 0:
@@ -264,6 +306,7 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": srv,
+        "beam_convolution": beam,
     }
     print(json.dumps(out))
     return 0
