@@ -9,10 +9,10 @@ HIP kernels reached through a C ABI (include/martini_mi355x.h).
 
 from .datacube import DataCube, HIfreq
 from .L_coords import L_coords
-from .martini import Martini
+from .martini import GlobalProfile, Martini
 from .sources import SPHSource
 from . import beams, sph_kernels, spectral_models
 
-__all__ = ["Martini", "DataCube", "SPHSource", "L_coords", "HIfreq", "beams", "sph_kernels",
+__all__ = ["Martini", "GlobalProfile", "DataCube", "SPHSource", "L_coords", "HIfreq", "beams", "sph_kernels",
            "spectral_models"]
 __version__ = "0.1.0"

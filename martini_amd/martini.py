@@ -48,6 +48,10 @@ class _BaseMartini:
         self.sph_kernel._init_sm_ranges()
         self._prune_particles(**(_prune_kwargs or {}))
 
+    @property
+    def datacube(self):
+        return self._datacube
+
     def init_spectra(self) -> None:
         """N x Nc spectra, for callers that want them (martini.py:153-166)."""
         if not self.quiet:
@@ -143,10 +147,6 @@ class Martini(_BaseMartini):
         super().__init__(source=source, datacube=datacube, beam=beam, noise=noise,
                          sph_kernel=sph_kernel, spectral_model=spectral_model, quiet=quiet)
 
-    @property
-    def datacube(self):
-        return self._datacube
-
     def convolve_beam(self) -> None:
         """Convolve every channel map with the beam image, drop the padding and
         convert Jy/arcsec^2 to Jy/beam (martini.py:863-901); on the GPU."""
@@ -193,3 +193,82 @@ class Martini(_BaseMartini):
         """
         self._insert_source_in_cube(skip_validation=skip_validation, progressbar=progressbar,
                                     ncpu=ncpu, method=method)
+
+
+class GlobalProfile(_BaseMartini):
+    """The spatially integrated spectrum of a source (martini.py:1369-1596).
+
+    Every particle in the spectral band contributes its whole line: a 1 x 1
+    pixel cube of 1 degree pixels (datacube.py:889-940), a Dirac kernel of
+    unbounded range, no spatial pruning, and every particle placed on the one
+    pixel.  The insertion itself is the same GPU path as ``Martini``'s.
+    """
+
+    def __init__(self, *, source, spectral_model, n_channels: int, channel_width: float,
+                 spectral_centre: float = 0.0, quiet: bool = False, channels=None,
+                 freq_channels: bool = False) -> None:
+        from .datacube import DataCube
+        from .sph_kernels import DiracDeltaKernel
+
+        if channels is not None:
+            warnings.warn(DeprecationWarning(
+                "The `channels` argument to `GlobalProfile.__init__` is deprecated and has "
+                "been ignored."))
+        if source.hsm_g is None:
+            source.hsm_g = np.zeros(source.npart)
+        dc = DataCube(n_px_x=1, n_px_y=1, n_channels=n_channels, px_size=3600.0,
+                      channel_width=channel_width, spectral_centre=spectral_centre,
+                      freq_channels=freq_channels)
+        super().__init__(source=source, datacube=dc, beam=None, noise=None,
+                         sph_kernel=DiracDeltaKernel(size_in_fwhm=np.inf),
+                         spectral_model=spectral_model, quiet=quiet,
+                         _prune_kwargs=dict(spatial=False, spectral=True, mass=True,
+                                            obj_type_str="spectrum"))
+        self.source.pixcoords[:2] = 0
+
+    def insert_source_in_spectrum(self, method: str = "plain") -> None:
+        """Fill the spectrum (Jy per channel); martini.py:1550-1596."""
+        super()._insert_source_in_cube(skip_validation=True, progressbar=False, ncpu=1,
+                                       quiet=True, method=method)
+        # Jy/arcsec^2 over the one pixel -> Jy
+        self._spectrum = np.asarray(self._datacube._array).squeeze() * self._datacube.px_size**2
+        if not self.quiet:
+            flux = self._spectrum.sum()
+            mass = (2.36e5 * self.source.distance**2 * flux
+                    * np.abs(np.diff(self._datacube.velocity_channel_edges)).mean() / 1e3)
+            print("Source inserted.", f"  Flux density in spectrum: {flux:.2e} Jy",
+                  f"  Mass in spectrum (assuming distance {self.source.distance:.2f} Mpc): "
+                  f"{mass:.2e} solMass",
+                  f"    [{mass / self.source.input_mass * 100:.0f}% of initial source mass]",
+                  sep="\n")
+
+    @property
+    def spectrum(self) -> np.ndarray:
+        """The spectrum in Jy, evaluated on first access."""
+        if not hasattr(self, "_spectrum"):
+            self.insert_source_in_spectrum()
+        return self._spectrum
+
+    @property
+    def channel_edges(self):
+        return self._datacube.channel_edges
+
+    @property
+    def channel_mids(self):
+        return self._datacube.channel_mids
+
+    @property
+    def velocity_channel_edges(self):
+        return self._datacube.velocity_channel_edges
+
+    @property
+    def velocity_channel_mids(self):
+        return self._datacube.velocity_channel_mids
+
+    @property
+    def frequency_channel_edges(self):
+        return self._datacube.frequency_channel_edges
+
+    @property
+    def frequency_channel_mids(self):
+        return self._datacube.frequency_channel_mids

@@ -681,3 +681,23 @@ def convolve_beam(cube, kernel, padx, pady):
     if padx > 0 and pady > 0:
         out = out[padx:-padx, pady:-pady, :]
     return out
+
+
+# -------------------------------------------------------------- global profile
+def global_profile(xyz_kpc, vxyz_kms, mHI, hsm_kpc, T_K, *, distance_mpc, n_channels,
+                   channel_width, spectral_centre, sigma="thermal", spectrum="gauss",
+                   rotation=None, h=0.7):
+    """GlobalProfile.insert_source_in_spectrum (martini/martini.py:1369-1596):
+    a 1-pixel cube of 1 degree pixels (datacube.py:889-940), every particle in
+    the spectral band contributing its whole line (no spatial aperture; the
+    prune keeps spectral and mass cuts only, martini.py:1536-1548), summed over
+    particles, in Jy per channel.  Pinned by the flux-sum identity of the
+    spectra (sum over channels = mHI / (2.36e5 d^2 |dv|) for lines inside the
+    band)."""
+    cube = CubeSpec(1, 1, n_channels, 3600.0, channel_width, spectral_centre=spectral_centre)
+    P = prepare(xyz_kpc, vxyz_kms, mHI, hsm_kpc, T_K, distance_mpc=distance_mpc, cube=cube,
+                kernel=KernelSpec("dirac", dirac_size=np.inf), sigma=sigma, spectrum=spectrum,
+                rotation=rotation, h=h, prune=False)
+    hw = P.sigma_kms if spectrum == "gauss" else 0.0
+    keep = prune_mask(P.pixcoords, P.sm_ranges, P.mHI, cube, hw, spatial=False)
+    return particle_spectra(P)[keep].sum(axis=0), int(keep.sum())

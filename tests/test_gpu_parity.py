@@ -95,3 +95,28 @@ def test_kernel_and_spectrum_kinds(case, method):
     print(f"{case} {method}: voxel {voxel:.3e} flux {flux:.3e}")
     assert voxel <= VOXEL_TOL
     assert flux <= FLUX_TOL
+
+
+@pytest.mark.parametrize("spectrum", ["gauss", "dirac"])
+@pytest.mark.parametrize("method", ["plain", "mfma"])
+def test_global_profile(spectrum, method):
+    """GlobalProfile's spatially integrated spectrum against the oracle."""
+    from martini_amd import GlobalProfile, SPHSource
+    from martini_amd.spectral_models import DiracDeltaSpectrum, GaussianSpectrum
+
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    a = w.arrays(4000)
+    src = SPHSource(distance=w.distance, rotation=w.rotation, h=0.7, T_g=a.T, mHI_g=a.mHI,
+                    xyz_g=a.xyz, vxyz_g=a.vxyz, hsm_g=a.hsm)
+    model = GaussianSpectrum(sigma="thermal") if spectrum == "gauss" else DiracDeltaSpectrum()
+    gp = GlobalProfile(source=src, spectral_model=model, n_channels=48, channel_width=6.0,
+                       spectral_centre=w.vsys, quiet=True)
+    gp.insert_source_in_spectrum(method=method)
+    ref, kept = O.global_profile(a.xyz, a.vxyz, a.mHI, a.hsm, a.T, distance_mpc=w.distance,
+                                 n_channels=48, channel_width=6.0, spectral_centre=w.vsys,
+                                 spectrum=spectrum, rotation=w.rotation.as_matrix())
+    err = np.abs(gp.spectrum - ref).max() / np.abs(ref).max()
+    flux = abs(gp.spectrum.sum() - ref.sum()) / ref.sum()
+    print(f"global profile {spectrum} {method}: max {err:.3e} flux {flux:.3e}, kept {kept}")
+    assert gp.source.npart == kept
+    assert err <= VOXEL_TOL and flux <= FLUX_TOL

@@ -296,3 +296,23 @@ def test_prepare_argument_errors():
     assert L.mi_prepare(*args) == -1
     o.n_tiers, o.spectrum = 3, 7
     assert L.mi_prepare(*args) == -1
+
+
+def test_global_profile_setup():
+    from martini_amd import GlobalProfile
+    from martini_amd.sph_kernels import DiracDeltaKernel
+
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    a = w.arrays(2000)
+    src = SPHSource(distance=w.distance, rotation=w.rotation, T_g=a.T, mHI_g=a.mHI, xyz_g=a.xyz,
+                    vxyz_g=a.vxyz, hsm_g=a.hsm)
+    gp = GlobalProfile(source=src, spectral_model=GaussianSpectrum(sigma="thermal"),
+                       n_channels=16, channel_width=4.0, spectral_centre=w.vsys, quiet=True)
+    assert gp.datacube._array.shape == (1, 1, 16) and gp.datacube.px_size == 3600.0
+    assert isinstance(gp.sph_kernel, DiracDeltaKernel)
+    assert np.all(np.isinf(gp.sph_kernel.sm_ranges))
+    assert np.all(gp.source.pixcoords[:2] == 0)
+    # spectral pruning only: a narrow band drops the particles outside it
+    assert 0 < gp.source.npart < 2000
+    rec, desc = _pack.pack_particles(gp.source, gp.sph_kernel, gp.spectral_model, gp.datacube)
+    assert np.all(rec["ix"] == 0) and np.all(rec["iy"] == 0) and np.all(rec["r_kind"] >> 8 == 0)
