@@ -166,17 +166,19 @@ __device__ inline float erf_diff(float a, float b) {
 // of the span's peak for D <= 0.5), which has no cancellation and costs one
 // hardware exp (v_exp_f32 of -m^2 log2 e: relative error a few 1e-7 for the
 // m^2 <= 25 that carry flux); wider spans take erf_diff.
+__device__ inline float span_series(float m, float D) {
+    const float m2 = m * m, D2 = D * D;
+    float p = 64.0f * m2 - 480.0f;
+    p = p * m2 + 720.0f;
+    p = p * m2 - 120.0f;                                   // H6-type polynomial
+    float q = (16.0f * m2 - 48.0f) * m2 + 12.0f;            // H4-type polynomial
+    float poly = 1.0f + D2 * ((4.0f * m2 - 2.0f) * (1.0f / 24.0f) +
+                              D2 * (q * (1.0f / 1920.0f) + D2 * p * (1.0f / 322560.0f)));
+    return 1.12837916709551257f * __expf(-m2) * D * poly;
+}
+
 __device__ inline float erf_span(float m, float D) {
-    if (D <= 0.5f) {
-        const float m2 = m * m, D2 = D * D;
-        float p = 64.0f * m2 - 480.0f;
-        p = p * m2 + 720.0f;
-        p = p * m2 - 120.0f;                                   // H6-type polynomial
-        float q = (16.0f * m2 - 48.0f) * m2 + 12.0f;            // H4-type polynomial
-        float poly = 1.0f + D2 * ((4.0f * m2 - 2.0f) * (1.0f / 24.0f) +
-                                  D2 * (q * (1.0f / 1920.0f) + D2 * p * (1.0f / 322560.0f)));
-        return 1.12837916709551257f * __expf(-m2) * D * poly;
-    }
+    if (D <= 0.5f) return span_series(m, D);
     return erf_diff(m - 0.5f * D, m + 0.5f * D);
 }
 

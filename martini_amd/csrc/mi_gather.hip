@@ -12,8 +12,8 @@
 //      transcendental work -- into a zeroed 32 x 256 LDS tile;
 //   3. S[k][channel] (line profile times amplitude) fills a 32 x 64 LDS tile;
 //   4. each thread owns one tile row (16 pixels) x 4 channels and accumulates
-//      64 FMAs per particle; a wave skips particles whose footprint misses its
-//      4 rows.
+//      up to 64 FMAs per particle; a wave skips particles whose footprint
+//      misses its 4 rows, and the 4-column groups the footprint misses.
 // Summation follows the sorted particle order: no atomics, deterministic.
 #include <hip/hip_runtime.h>
 
@@ -22,6 +22,8 @@
 #include "mi_tile.h"
 
 namespace mi {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Three waves per SIMD (at most 168 VGPRs): three work groups share a CU.
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_gather_plain(
@@ -83,18 +85,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         const int klo = cs.ilo[lane & (KC - 1)], khi = cs.ihi[lane & (KC - 1)];
         const bool hit = lane < KC && klo <= khi && khi >= wrow0 && klo <= wrow1;
         uint32_t kmask = (uint32_t)__ballot(hit);
+        // Lane k: the 4-column groups of the tile that particle k's footprint
+        // covers (W is zero in the others).  Read here, before the next chunk's
+        // staging may overwrite the chunk state.
+        uint32_t qm = 0;
+        if (hit) {
+            const WalkRec& wk = cs.walk[lane];
+            const int j0 = (wk.ij >> 16) - y0;
+            qm = (2u << ((j0 + (wk.kw >> 8) - 1) >> 2)) - (1u << (j0 >> 2));
+        }
         fill_weights(cs, x0, y0, wtab, tkind, Tk, [&](int k, int px, float w) { Wt[k][px] = w; });
-        fill_spectra(cs, nk, c0, g.nc, d.spectrum, [&](int k, int cl, float v) { St[k][cl] = v; });
+        fill_spectra_rows(cs, nk, c0, g.nc, d.spectrum, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
         if (kmask == 0u) continue;
         while (kmask != 0u) {
             const int k = __builtin_ctz(kmask);
             kmask &= kmask - 1u;
-            const float4* wr = reinterpret_cast<const float4*>(&Wt[k][pg * TILE]);
-            const float4 sv = *reinterpret_cast<const float4*>(&St[k][cg * 4]);
+            const uint32_t qk = __builtin_amdgcn_readlane(qm, k);  // wave-uniform
+            const f32x4* wr = reinterpret_cast<const f32x4*>(&Wt[k][pg * TILE]);
+            const f32x4 sv = *reinterpret_cast<const f32x4*>(&St[k][cg * 4]);
+            // all five LDS reads are issued together, ahead of the skips
+            f32x4 wq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wq[q] = wr[q];
+            asm volatile("" ::"v"(wq[0]), "v"(wq[1]), "v"(wq[2]), "v"(wq[3]), "v"(sv));
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float4 w = wr[q];
+                if (!((qk >> q) & 1u)) continue;
+                const f32x4 w = wq[q];
                 const float wv[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {

@@ -164,9 +164,12 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
 
 // Evaluates S for the chunk's KC particles x CBLOCK channels of the cell and
 // hands each value to store(k, channel-in-block, s); particles k >= nk and
-// channels beyond the cube give 0.  Thread t takes particle t % KC, so its
-// record fields are read once.  Only the 16-channel blocks set in `blocks`
-// (wave-uniform) are evaluated; the others are left untouched.
+// channels beyond the cube give 0.  Thread t takes particle t % KC and the
+// channels t / KC + 8 i, so its record fields are read once.  Only the
+// 16-channel blocks set in `blocks` (wave-uniform) are evaluated; the others
+// are left untouched.  The values are line_flux's, computed as straight-line
+// code (the window test is a select, the wide-channel erfc path one branch
+// per block) so that the evaluations of a thread overlap.
 template <class Store>
 __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc, int spectrum,
                                     Store store, uint32_t blocks = 0xFu) {
@@ -174,14 +177,56 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     const int cl0 = threadIdx.x / KC;
     const mi_particle& p = cs.p[k];
     const bool live = k < nk;
-    const int lo = live ? p.c_lo : 1, hi = live ? p.c_hi : 0;
+    const int lo = live ? p.c_lo : 1, hi = live ? min(p.c_hi, nc - 1) : 0;
     const int k0 = p.k0;
     const float fu = p.fu, s = p.s, amp = p.amp;
+    const bool dirac = spectrum == MI_SPECTRUM_DIRAC;
+    constexpr int PER_BLOCK = 16 / (NT / KC);  // a thread's channels in one 16-channel block
 #pragma unroll
-    for (int cl = cl0; cl < CBLOCK; cl += NT / KC) {
-        if (!((blocks >> (cl >> 4)) & 1u)) continue;  // uniform: a wave's lanes share cl >> 4
-        const int c = c0 + cl;
-        store(k, cl, c < nc ? line_flux(lo, hi, k0, fu, s, amp, c, spectrum) : 0.0f);
+    for (int b = 0; b < CBLOCK / 16; ++b) {
+        if (!((blocks >> b) & 1u)) continue;  // uniform
+        float v[PER_BLOCK];
+#pragma unroll
+        for (int i = 0; i < PER_BLOCK; ++i) {
+            const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK + i);
+            const float dd = (float)(k0 - c) + fu;
+            v[i] = span_series(s * (dd - 0.5f), s);
+        }
+        if (!dirac && s > 0.5f) {
+#pragma unroll
+            for (int i = 0; i < PER_BLOCK; ++i) {
+                const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK + i);
+                const float m = s * (((float)(k0 - c) + fu) - 0.5f);
+                v[i] = erf_diff(m - 0.5f * s, m + 0.5f * s);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PER_BLOCK; ++i) {
+            const int cl = cl0 + (NT / KC) * (b * PER_BLOCK + i);
+            const int c = c0 + cl;
+            const float f = dirac ? amp : 0.5f * amp * v[i];
+            store(k, cl, (c >= lo && c <= hi) ? f : 0.0f);
+        }
+    }
+}
+
+// The same values as fill_spectra, with the lanes of a wave over the CBLOCK
+// channels of one particle (wave w takes particles w, w + 4, ...): a row of
+// an [particle][channel] tile is written by consecutive lanes, free of LDS
+// bank conflicts, and the particle's fields are read once per wave.
+template <class Store>
+__device__ inline void fill_spectra_rows(const ChunkState& cs, int nk, int c0, int nc,
+                                         int spectrum, Store store) {
+    const int cl = threadIdx.x & (CBLOCK - 1);
+    const int c = c0 + cl;
+#pragma unroll 2
+    for (int k = threadIdx.x / CBLOCK; k < KC; k += NT / CBLOCK) {
+        float v = 0.0f;
+        if (k < nk && c < nc) {
+            const mi_particle& p = cs.p[k];
+            v = line_flux(p.c_lo, p.c_hi, p.k0, p.fu, p.s, p.amp, c, spectrum);
+        }
+        store(k, cl, v);
     }
 }
 
