@@ -36,6 +36,8 @@ struct ChunkState {
     int ilo[KC], ihi[KC];
     int wdiv[KC];       // ceil(2^16 / footprint width): (n * wdiv) >> 16 == n / width for n < 4096
     int off[KC + 1];    // exclusive scan of the per-particle pair counts
+    int toff[KC + 1];   // exclusive scan of the threads given to each particle's pairs
+    int per;            // pairs per thread
     uint32_t rowmask;   // tile rows (x offsets) touched by any footprint of the chunk
     uint32_t chanmask;  // 16-channel sub-blocks of the cell touched by any window
 };
@@ -76,6 +78,16 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
         rows |= __shfl_xor(rows, dd, 64);
         chans |= __shfl_xor(chans, dd, 64);
     }
+    // Each particle's pairs go to ceil(cnt / per) threads of their own, per =
+    // ceil(total / (NT - KC)): at most total / per + KC <= NT threads in all.
+    const int total = __shfl(incl, 63, 64);
+    const int per = max(1, (total + (NT - KC) - 1) / (NT - KC));
+    int tincl = (cnt + per - 1) / per;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int v = __shfl_up(tincl, dd, 64);
+        if (lane >= dd) tincl += v;
+    }
     if (lane < KC) {
         if (valid) cs.p[lane] = p;
         cs.ilo[lane] = ilo;
@@ -83,6 +95,7 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
         const int width = max(jhi - jlo + 1, 1);
         cs.wdiv[lane] = (65536 + width - 1) / width;
         cs.off[lane + 1] = incl;
+        cs.toff[lane + 1] = tincl;
         if (cnt > 0) {
             const int kind = kind_of(p.r_kind), gi = gidx_of(p.r_kind);
             const WFactors f = weight_factors(kind, p.h, d.gauss_t[gi], d.gauss_inv_norm[gi]);
@@ -101,59 +114,95 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
     }
     if (lane == 0) {
         cs.off[0] = 0;
+        cs.toff[0] = 0;
+        cs.per = per;
         cs.rowmask = rows;
         cs.chanmask = chans;
     }
 }
 
+// The weight of a tabulated kernel at q^2 = dr^2 / h^2, as weight_at computes
+// it, without branches: fb = 1/h^2, tab the kernel's table, cubic whether the
+// seam term of the cubic spline applies.
+__device__ inline float table_weight_select(const float4* __restrict__ tab, float q2, float fb,
+                                            bool cubic) {
+    const float q = __builtin_amdgcn_sqrtf(q2);
+    const float x = q * (float)WTAB_N;
+    const int i = min((int)x, WTAB_N - 1);
+    const float t = x - (float)i;
+    const float4 c = tab[i];
+    float w = q < 1.0f ? fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x) : 0.0f;
+    if (cubic) {
+        const float R2 = 4.0f * q2;
+        const float seam =
+            R2 * __builtin_amdgcn_sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
+        w = q2 <= 0.25f ? w + seam : w;
+    }
+    return w * fb;
+}
+
 // Evaluates W for every (particle, pixel) pair of the chunk and hands each
-// value to store(k, pixel-in-tile, w).  Pair q belongs to the particle k with
-// off[k] <= q < off[k+1]; its pixel is row q' / width, column q' % width of
-// the clipped footprint, q' = q - off[k] (width <= 16 and q' < 256, where the
-// reciprocal multiply is exact).  Thread t takes the contiguous pairs
-// [t*per, (t+1)*per): one binary search, then an incremental walk that keeps
-// the particle's values in registers until its pairs run out.
+// value to store(k, pixel-in-tile, w).  Particle k's pairs (pixel row l / width,
+// column l % width of its clipped footprint, l < cnt) are split over threads
+// toff[k] .. toff[k+1] - 1, `per` consecutive pairs each, so a thread stays on
+// one particle: one binary search, then a walk over its pixels.  When every
+// particle of the wave's pairs uses the item's LDS table, the walk is
+// branch-free and takes two pixels per step, whose evaluations overlap.
 template <class Store>
 __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                                     const float4* __restrict__ wtab, int tab_kind,
                                     const float4* __restrict__ tab, Store store) {
-    const int total = cs.off[KC];
-    const int per = (total + NT - 1) / NT;
-    int q = threadIdx.x * per;
-    const int qend = min(q + per, total);
-    if (q >= qend) return;
+    const int t = threadIdx.x;
+    const bool active = t < cs.toff[KC];
     int k = 0;
+    if (active) {
 #pragma unroll
-    for (int step = KC / 2; step > 0; step >>= 1)
-        if (cs.off[k + step] <= q) k += step;
-    int kend = cs.off[k + 1];
-    WalkRec w = cs.walk[k];
-    int width = w.kw >> 8;
-    const int local = q - cs.off[k];
-    int a = (local * cs.wdiv[k]) >> 16;
-    int b = local - a * width;
-    for (; q < qend; ++q) {
-        if (q >= kend) {  // next particle that has pairs
-            do {
-                ++k;
-                kend = cs.off[k + 1];
-            } while (q >= kend);
-            w = cs.walk[k];
-            width = w.kw >> 8;
-            a = 0;
-            b = 0;
+        for (int step = KC / 2; step > 0; step >>= 1)
+            if (cs.toff[k + step] <= t) k += step;
+    }
+    const WalkRec w = cs.walk[k];
+    const int width = w.kw >> 8, kind = w.kw & 255;
+    const int l0 = (t - cs.toff[k]) * cs.per;
+    const int l1 = active ? min(l0 + cs.per, cs.off[k + 1] - cs.off[k]) : l0;
+    int a = (l0 * cs.wdiv[k]) >> 16;
+    int b = l0 - a * width;
+    const int ti = (w.ij & 0xffff) - x0, tj = (w.ij >> 16) - y0;  // footprint corner in the tile
+    const double px = w.px - (double)x0, py = w.py - (double)y0;   // particle in tile coordinates
+
+    if (tab_kind >= 0 && __all(!active || kind == tab_kind)) {
+        const float fb = (float)w.fa;
+        const bool cubic = tab_kind == MI_KERNEL_CUBIC;
+        auto eval = [&](int ra, int rb) {
+            const double dx = px - (double)(ti + ra), dy = py - (double)(tj + rb);
+            return table_weight_select(tab, (float)((dx * dx + dy * dy) * w.fa), fb, cubic);
+        };
+        int l = l0;
+        for (; l + 1 < l1; l += 2) {
+            const int a1 = a, b1 = b;
+            b = b + 1 == width ? 0 : b + 1;
+            a = b == 0 ? a + 1 : a;
+            const int a2 = a, b2 = b;
+            b = b + 1 == width ? 0 : b + 1;
+            a = b == 0 ? a + 1 : a;
+            const float v1 = eval(a1, b1), v2 = eval(a2, b2);
+            store(k, (ti + a1) * TILE + tj + b1, v1);
+            store(k, (ti + a2) * TILE + tj + b2, v2);
         }
-        const int kind = w.kw & 255;
-        const int i = (w.ij & 0xffff) + a, j = (w.ij >> 16) + b;
-        WFactors f{w.fa, 0.0, (float)w.fa, 0.0f, 0.0f};
-        if (kind == MI_KERNEL_GAUSS) {
-            const float4 g = cs.gf[k];
-            f.e = cs.ge[k];
-            f.b = g.x;
-            f.t = g.y;
-            f.n = g.z;
-        }
-        store(k, (i - x0) * TILE + (j - y0),
+        if (l < l1) store(k, (ti + a) * TILE + tj + b, eval(a, b));
+        return;
+    }
+
+    WFactors f{w.fa, 0.0, (float)w.fa, 0.0f, 0.0f};
+    if (kind == MI_KERNEL_GAUSS) {
+        const float4 gg = cs.gf[k];
+        f.e = cs.ge[k];
+        f.b = gg.x;
+        f.t = gg.y;
+        f.n = gg.z;
+    }
+    for (int l = l0; l < l1; ++l) {
+        const int i = x0 + ti + a, j = y0 + tj + b;
+        store(k, (ti + a) * TILE + tj + b,
               weight_at(kind, w.px - (double)i, w.py - (double)j, f, wtab, tab_kind, tab));
         if (++b == width) {
             b = 0;
