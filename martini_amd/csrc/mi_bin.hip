@@ -53,16 +53,13 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     L->vals2 = take(4 * np);
     L->cell_start = take(4 * (size_t)g.ncells);
     L->cell_end = take(4 * (size_t)g.ncells);
-    L->item_counts = take(4 * nc1);
     L->item_off = take(4 * nc1);
-    L->multi_counts = take(4 * nc1);
     L->multi_off = take(4 * nc1);
     L->items = take(sizeof(Item) * (size_t)L->items_max);
     L->partials = take(sizeof(float) * (size_t)L->multi_max * TILE_PX * CBLOCK);
     L->status = take(4 * 8);
-    L->wtab = take(sizeof(float4) * WTAB_KERNELS * WTAB_N);
-    // rocPRIM temporary storage: the largest of the three uses.
-    size_t t1 = 0, t2 = 0, t3 = 0;
+    // rocPRIM temporary storage: the larger of its two uses.
+    size_t t1 = 0, t2 = 0;
     uint32_t* nul = nullptr;
     hipError_t e = rocprim::exclusive_scan(nullptr, t1, nul, nul, 0u, nn,
                                            rocprim::plus<uint32_t>(), 0);
@@ -70,10 +67,7 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     e = rocprim::radix_sort_pairs(nullptr, t2, nul, nul, nul, nul, np, 0,
                                   g.cell_bits + KEY_LOCAL_BITS, 0);
     if (e != hipSuccess) return e;
-    e = rocprim::exclusive_scan(nullptr, t3, nul, nul, 0u, nc1, rocprim::plus<uint32_t>(), 0);
-    if (e != hipSuccess) return e;
-    size_t tb = t1 > t2 ? t1 : t2;
-    tb = tb > t3 ? tb : t3;
+    const size_t tb = t1 > t2 ? t1 : t2;
     L->temp_bytes = tb;
     L->temp = take(tb);
     L->total = off;
@@ -142,11 +136,16 @@ __global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap
     }
 }
 
-__global__ void k_fill_wtab(float4* __restrict__ tab) { fill_wtab(tab); }
-
-__global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
+// Zeroes the status words and the cell bounds (cells without particles keep
+// start = end = 0).
+__global__ void k_bin_init(uint32_t* __restrict__ status, uint32_t* __restrict__ start,
+                           uint32_t* __restrict__ end, int64_t ncells) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) a[k] = v;
+    if (k < 8) status[k] = 0u;
+    if (k < ncells) {
+        start[k] = 0u;
+        end[k] = 0u;
+    }
 }
 
 __global__ void k_cell_bounds(const uint32_t* __restrict__ keys, int64_t np, int64_t ncells,
@@ -161,41 +160,79 @@ __global__ void k_cell_bounds(const uint32_t* __restrict__ keys, int64_t np, int
     if (c != next) end[c] = (uint32_t)(q + 1);
 }
 
-__global__ void k_item_counts(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                              int64_t ncells, int item_len, const uint32_t* __restrict__ status,
-                              uint32_t* __restrict__ item_counts,
-                              uint32_t* __restrict__ multi_counts) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > ncells) return;
-    uint32_t ni = 0;
-    if (c < ncells && !status[ST_OVERFLOW] && end[c] > start[c]) {
-        uint32_t len = end[c] - start[c];
-        ni = (len + item_len - 1) / item_len;
+// Work items per cell (ceil(len / item_len), none after an overflow) and
+// their exclusive scans, in one work group: thread t takes a contiguous run of
+// cells, the run totals are scanned in LDS, and the offsets written.
+// item_off[ncells] and multi_off[ncells] hold the totals (also status words).
+constexpr int SCAN_T = 1024;
+
+__global__ __launch_bounds__(SCAN_T) void k_item_scan(const uint32_t* __restrict__ start,
+                                                      const uint32_t* __restrict__ end,
+                                                      int64_t ncells, int item_len,
+                                                      uint32_t* __restrict__ status,
+                                                      uint32_t* __restrict__ item_off,
+                                                      uint32_t* __restrict__ multi_off) {
+    __shared__ uint32_t si[SCAN_T], sm[SCAN_T];
+    const int t = threadIdx.x;
+    const int64_t per = (ncells + SCAN_T - 1) / SCAN_T;
+    const int64_t c0 = min((int64_t)t * per, ncells), c1 = min(c0 + per, ncells);
+    const bool overflow = status[ST_OVERFLOW] != 0u;
+    const uint32_t L = (uint32_t)item_len;
+    auto items_of = [&](int64_t c) {
+        const uint32_t s0 = start[c], e0 = end[c];
+        return (overflow || e0 <= s0) ? 0u : (e0 - s0 + L - 1) / L;
+    };
+    uint32_t ai = 0, am = 0;
+    for (int64_t c = c0; c < c1; ++c) {
+        const uint32_t ni = items_of(c);
+        ai += ni;
+        am += ni > 1 ? ni : 0u;
     }
-    item_counts[c] = ni;
-    multi_counts[c] = ni > 1 ? ni : 0;
+    si[t] = ai;
+    sm[t] = am;
+    __syncthreads();
+    for (int d = 1; d < SCAN_T; d <<= 1) {  // inclusive scan of the run totals
+        const uint32_t vi = t >= d ? si[t - d] : 0u, vm = t >= d ? sm[t - d] : 0u;
+        __syncthreads();
+        si[t] += vi;
+        sm[t] += vm;
+        __syncthreads();
+    }
+    uint32_t bi = si[t] - ai, bm = sm[t] - am;
+    for (int64_t c = c0; c < c1; ++c) {
+        const uint32_t ni = items_of(c);
+        item_off[c] = bi;
+        multi_off[c] = bm;
+        bi += ni;
+        bm += ni > 1 ? ni : 0u;
+    }
+    if (t == SCAN_T - 1) {
+        item_off[ncells] = si[t];
+        multi_off[ncells] = sm[t];
+        status[ST_ITEMS] = si[t];
+        status[ST_MULTI] = sm[t];
+    }
 }
 
-__global__ void k_build_items(const uint32_t* __restrict__ start, const uint32_t* __restrict__ end,
-                              int64_t ncells, int item_len, const uint32_t* __restrict__ item_off,
-                              const uint32_t* __restrict__ multi_off, Item* __restrict__ items,
-                              uint32_t* __restrict__ status) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c == 0) {
-        status[ST_ITEMS] = item_off[ncells];
-        status[ST_MULTI] = multi_off[ncells];
-    }
-    if (c >= ncells) return;
-    uint32_t s0 = start[c], e0 = end[c];
-    const uint32_t L = (uint32_t)item_len;
-    uint32_t ni = (status[ST_OVERFLOW] || e0 <= s0) ? 0u : (e0 - s0 + L - 1) / L;
-    for (uint32_t q = 0; q < ni; ++q) {
+// One work group per cell writes the cell's items; a cell cut into several
+// items gives each a partial-tile slot.
+__global__ __launch_bounds__(64) void k_build_items(const uint32_t* __restrict__ start,
+                                                    const uint32_t* __restrict__ end,
+                                                    int item_len,
+                                                    const uint32_t* __restrict__ item_off,
+                                                    const uint32_t* __restrict__ multi_off,
+                                                    Item* __restrict__ items) {
+    const uint32_t c = blockIdx.x;
+    const uint32_t base = item_off[c], ni = item_off[c + 1] - base;
+    if (ni == 0u) return;
+    const uint32_t s0 = start[c], e0 = end[c], L = (uint32_t)item_len;
+    for (uint32_t q = threadIdx.x; q < ni; q += blockDim.x) {
         Item it;
         it.cell = (int32_t)c;
         it.begin = (int32_t)(s0 + q * L);
         it.end = (int32_t)min(e0, s0 + (q + 1) * L);
         it.slot = ni > 1 ? (int32_t)(multi_off[c] + q) : -1;
-        items[item_off[c] + q] = it;
+        items[base + q] = it;
     }
 }
 
@@ -211,9 +248,7 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
     uint32_t* vals2 = (uint32_t*)(ws + L.vals2);
     uint32_t* cstart = (uint32_t*)(ws + L.cell_start);
     uint32_t* cend = (uint32_t*)(ws + L.cell_end);
-    uint32_t* icount = (uint32_t*)(ws + L.item_counts);
     uint32_t* ioff = (uint32_t*)(ws + L.item_off);
-    uint32_t* mcount = (uint32_t*)(ws + L.multi_counts);
     uint32_t* moff = (uint32_t*)(ws + L.multi_off);
     Item* items = (Item*)(ws + L.items);
     uint32_t* status = (uint32_t*)(ws + L.status);
@@ -221,9 +256,8 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
     size_t tb = L.temp_bytes;
     const int T = 256;
 
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(T), 0, s, status, (int64_t)8, 0u);
-    hipLaunchKernelGGL(k_fill_wtab, dim3((WTAB_KERNELS * WTAB_N + T - 1) / T), dim3(T), 0, s,
-                       (float4*)(ws + L.wtab));
+    hipLaunchKernelGGL(k_bin_init, dim3(blocks_for(g.ncells > 8 ? g.ncells : 8, T)), dim3(T), 0, s,
+                       status, cstart, cend, g.ncells);
     if (n > 0) {
         hipLaunchKernelGGL(k_count, dim3(blocks_for(n, T)), dim3(T), 0, s, d_p, n, g, counts);
         hipError_t e = rocprim::exclusive_scan(temp, tb, counts, offsets, 0u, (size_t)n,
@@ -242,21 +276,13 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
                                                  g.cell_bits + KEY_LOCAL_BITS, s);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cstart, g.ncells, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cend, g.ncells, 0u);
     if (n_pairs > 0)
         hipLaunchKernelGGL(k_cell_bounds, dim3(blocks_for(n_pairs, T)), dim3(T), 0, s, keys2,
                            n_pairs, g.ncells, cstart, cend);
-    hipLaunchKernelGGL(k_item_counts, dim3(blocks_for(g.ncells + 1, T)), dim3(T), 0, s, cstart,
-                       cend, g.ncells, L.item_len, status, icount, mcount);
-    hipError_t e = rocprim::exclusive_scan(temp, tb, icount, ioff, 0u, (size_t)g.ncells + 1,
-                                           rocprim::plus<uint32_t>(), s);
-    if (e != hipSuccess) return e;
-    e = rocprim::exclusive_scan(temp, tb, mcount, moff, 0u, (size_t)g.ncells + 1,
-                                rocprim::plus<uint32_t>(), s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_build_items, dim3(blocks_for(g.ncells, T)), dim3(T), 0, s, cstart, cend,
-                       g.ncells, L.item_len, ioff, moff, items, status);
+    hipLaunchKernelGGL(k_item_scan, dim3(1), dim3(SCAN_T), 0, s, cstart, cend, g.ncells,
+                       L.item_len, status, ioff, moff);
+    hipLaunchKernelGGL(k_build_items, dim3((unsigned)g.ncells), dim3(64), 0, s, cstart, cend,
+                       L.item_len, ioff, moff, items);
     return hipGetLastError();
 }
 

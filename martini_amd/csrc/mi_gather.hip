@@ -17,6 +17,8 @@
 // Summation follows the sorted particle order: no atomics, deterministic.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "mi_device.h"
 #include "mi_internal.h"
 #include "mi_tile.h"
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
 // independent.
 constexpr int RED_PARTS = TILE_PX * CBLOCK / (NT * 4);
 
-__global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item_counts,
+__global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item_off,
                                                const uint32_t* __restrict__ multi_off,
                                                const uint32_t* __restrict__ status, Grid g,
                                                const float* __restrict__ partials,
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item
     if (status[ST_OVERFLOW]) return;
     const int cell = blockIdx.x / RED_PARTS;
     const int part = blockIdx.x - cell * RED_PARTS;
-    const uint32_t ni = item_counts[cell];
+    const uint32_t ni = item_off[cell + 1] - item_off[cell];
     if (ni <= 1) return;
     const size_t tile = (size_t)TILE_PX * CBLOCK / 4;  // float4 per partial tile
     const float4* base = reinterpret_cast<const float4*>(partials) + (size_t)multi_off[cell] * tile;
@@ -198,18 +200,46 @@ __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item
         if (cp.c0 + cl + u < g.nc) dst[u] += sv[u];
 }
 
-// Weight table of the plugin-level entry point (mi_kernel_weights), refilled
-// before each of its launches on the same stream.
-__device__ float4 g_api_wtab[WTAB_KERNELS * WTAB_N];
+// The weight tables are constant data (the FP64 closed forms of the four
+// tabulated kernels), so each device holds one copy, filled on first use.
+__device__ float4 g_wtab[WTAB_KERNELS * WTAB_N];
 
-__global__ void k_fill_api_wtab() { fill_wtab(g_api_wtab); }
+__global__ void k_fill_wtab() { fill_wtab(g_wtab); }
+
+constexpr int MAX_DEVICES = 64;
+
+hipError_t weight_table(const float4** out) {
+    static std::mutex mu;
+    static const float4* table[MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (table[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_wtab));
+        if (e != hipSuccess) return e;
+        hipStream_t st;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_fill_wtab, dim3((WTAB_KERNELS * WTAB_N + 255) / 256), dim3(256), 0, st);
+        e = hipGetLastError();
+        const hipError_t es = hipStreamSynchronize(st);
+        hipStreamDestroy(st);
+        if (e != hipSuccess) return e;
+        if (es != hipSuccess) return es;
+        table[dev] = static_cast<const float4*>(p);
+    }
+    *out = table[dev];
+    return hipSuccess;
+}
 
 __global__ void k_weights(int kind, float gt, float ginv, const float* __restrict__ dx,
                           const float* __restrict__ dy, const float* __restrict__ h,
-                          float* __restrict__ out, int64_t n) {
+                          const float4* __restrict__ wtab, float* __restrict__ out, int64_t n) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n)
-        out[k] = pixel_weight(kind, (double)dx[k], (double)dy[k], h[k], gt, ginv, g_api_wtab);
+    if (k < n) out[k] = pixel_weight(kind, (double)dx[k], (double)dy[k], h[k], gt, ginv, wtab);
 }
 
 __global__ void k_spectra(const mi_particle* __restrict__ P, int64_t n, int nc, int spectrum,
@@ -224,9 +254,12 @@ __global__ void k_spectra(const mi_particle* __restrict__ P, int64_t n, int nc, 
 hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
                         const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
+    const float4* wtab = nullptr;
+    hipError_t e = weight_table(&wtab);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gather_plain, dim3((unsigned)L.items_max), dim3(NT), 0, s,
                        d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
-                       (const uint32_t*)(ws + L.status), (const float4*)(ws + L.wtab), g, d,
+                       (const uint32_t*)(ws + L.status), wtab, g, d,
                        d_cube, (float*)(ws + L.partials));
     return hipGetLastError();
 }
@@ -234,7 +267,7 @@ hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_des
 hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cube,
                            hipStream_t s) {
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)(g.ncells * RED_PARTS)), dim3(NT), 0, s,
-                       (const uint32_t*)(ws + L.item_counts), (const uint32_t*)(ws + L.multi_off),
+                       (const uint32_t*)(ws + L.item_off), (const uint32_t*)(ws + L.multi_off),
                        (const uint32_t*)(ws + L.status), g, (const float*)(ws + L.partials),
                        d_cube);
     return hipGetLastError();
@@ -243,9 +276,11 @@ hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cu
 hipError_t launch_weights(int kind, float gt, float ginv, const float* dx, const float* dy,
                           const float* h, float* out, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_api_wtab, dim3((WTAB_KERNELS * WTAB_N + 255) / 256), dim3(256), 0, s);
+    const float4* wtab = nullptr;
+    hipError_t e = weight_table(&wtab);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kind, gt,
-                       ginv, dx, dy, h, out, n);
+                       ginv, dx, dy, h, wtab, out, n);
     return hipGetLastError();
 }
 

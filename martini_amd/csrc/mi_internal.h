@@ -43,12 +43,11 @@ struct Layout {
     size_t counts, offsets;            // uint32[n] each
     size_t keys, vals, keys2, vals2;   // uint32[n_pairs] each; vals2: particle index in cell order
     size_t cell_start, cell_end;       // uint32[ncells] each
-    size_t item_counts, item_off;      // uint32[ncells + 1] each
-    size_t multi_counts, multi_off;    // uint32[ncells + 1] each
+    size_t item_off, multi_off;        // uint32[ncells + 1] each: exclusive scans of the
+                                       // items and the partial-tile slots per cell
     size_t items;                      // int4[items_max]
     size_t partials;                   // float[multi_max * TILE_PX * CBLOCK]
     size_t status;                     // uint32[8]
-    size_t wtab;                       // float4[WTAB_KERNELS * WTAB_N] weight table
     size_t temp;                       // rocPRIM temporary storage
     size_t temp_bytes;
     size_t total;
@@ -73,6 +72,9 @@ hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc
 hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cube,
                            hipStream_t s);
 
+// The per-device weight tables (WTAB_KERNELS x WTAB_N, filled by fill_wtab on
+// the device's first use, synchronously on a stream of their own).
+hipError_t weight_table(const float4** out);
 hipError_t launch_weights(int kind, float gt, float ginv, const float* dx, const float* dy,
                           const float* h, float* out, int64_t n, hipStream_t s);
 hipError_t launch_spectra(const mi_particle* d_p, int64_t n, const mi_cube_desc& d, float* out,

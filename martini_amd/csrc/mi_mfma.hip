@@ -174,9 +174,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
 hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
                        const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
+    const float4* wtab = nullptr;
+    hipError_t e = weight_table(&wtab);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gather_mfma, dim3((unsigned)L.items_max), dim3(NT), 0, s,
                        d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
-                       (const uint32_t*)(ws + L.status), (const float4*)(ws + L.wtab), g, d,
+                       (const uint32_t*)(ws + L.status), wtab, g, d,
                        d_cube, (float*)(ws + L.partials));
     return hipGetLastError();
 }
