@@ -244,6 +244,21 @@ int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
 int mi_convolve_beam(const float* d_in, float* d_out, int32_t nx, int32_t ny, int32_t nc,
                      const float* d_kernel, int32_t kx, int32_t ky, void* stream);
 
+/*
+ * The FITS data unit of a cube (Martini.write_fits, martini/martini.py:939-1055):
+ * the region [padx, padx + nx) x [pady, pady + ny) of the padded cube d_cube
+ * ([nx + 2 padx][ny + 2 pady][nc], FP32 when cube_bits is 32, FP64 when 64),
+ * written to d_out in FITS axis order (x fastest, then y, then channel: the
+ * reference's array transpose, :1049) as big-endian IEEE float32 (bitpix -32)
+ * or float64 (bitpix -64, the reference's default for its FP64 cube),
+ * nx * ny * nc * |bitpix| / 8 bytes.  d_range (2 doubles) receives the
+ * minimum and the maximum (DATAMIN, DATAMAX).  Asynchronous.  MI_ERR_LIMIT
+ * for nx or ny above 32767.
+ */
+int mi_cube_to_fits(const void* d_cube, int32_t cube_bits, int32_t nx, int32_t ny, int32_t nc,
+                    int32_t padx, int32_t pady, int32_t bitpix, void* d_out, double* d_range,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,8 @@ kernel and line integrals, and the insertion contraction run in hand-written
 HIP kernels reached through a C ABI (include/martini_mi355x.h).
 """
 
+__version__ = "0.1.0"
+
 from .datacube import DataCube, HIfreq
 from .L_coords import L_coords
 from .martini import GlobalProfile, Martini
@@ -15,4 +17,3 @@ from . import beams, sph_kernels, spectral_models
 
 __all__ = ["Martini", "GlobalProfile", "DataCube", "SPHSource", "L_coords", "HIfreq", "beams", "sph_kernels",
            "spectral_models"]
-__version__ = "0.1.0"

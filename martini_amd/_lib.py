@@ -115,6 +115,10 @@ def lib() -> ctypes.CDLL:
     L.mi_convolve_beam.restype = ctypes.c_int
     L.mi_convolve_beam.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp,
                                    ctypes.c_int32, ctypes.c_int32, vp]
+    L.mi_cube_to_fits.restype = ctypes.c_int
+    L.mi_cube_to_fits.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                  vp, vp, vp]
     L.mi_prepare_workspace_size.restype = ctypes.c_int
     L.mi_prepare_workspace_size.argtypes = [i64, ctypes.POINTER(ctypes.c_size_t)]
     L.mi_prepare.restype = ctypes.c_int
@@ -263,3 +267,15 @@ def convolve_beam(d_in, d_out, nx: int, ny: int, nc: int, d_kernel, kx: int, ky:
     check(lib().mi_convolve_beam(_ptr(d_in), _ptr(d_out), int(nx), int(ny), int(nc),
                                  _ptr(d_kernel), int(kx), int(ky), stream_ptr(stream)),
           "mi_convolve_beam")
+
+
+def cube_to_fits(d_cube, nx: int, ny: int, nc: int, padx: int, pady: int, bitpix: int, d_out,
+                 d_range, stream=None) -> None:
+    """Asynchronous FITS data unit of a device cube (mi_cube_to_fits): the
+    unpadded region in FITS axis order, big-endian, and its min and max.
+    d_cube is an FP32 or FP64 tensor; d_range two FP64 values."""
+    bits = 8 * d_cube.element_size()
+    check(lib().mi_cube_to_fits(_ptr(d_cube), bits, int(nx), int(ny), int(nc), int(padx),
+                                int(pady), int(bitpix), _ptr(d_out), _ptr(d_range),
+                                stream_ptr(stream)),
+          "mi_cube_to_fits")

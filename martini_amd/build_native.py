@@ -26,7 +26,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["mi_bin.hip", "mi_gather.hip", "mi_mfma.hip", "mi_prepare.hip", "mi_beam.hip",
-           "mi_capi.hip"]
+           "mi_output.hip", "mi_capi.hip"]
 
 
 def _compile(src: str) -> str:

@@ -225,4 +225,18 @@ int mi_convolve_beam(const float* d_in, float* d_out, int32_t nx, int32_t ny, in
     return MI_OK;
 }
 
+int mi_cube_to_fits(const void* d_cube, int32_t cube_bits, int32_t nx, int32_t ny, int32_t nc,
+                    int32_t padx, int32_t pady, int32_t bitpix, void* d_out, double* d_range,
+                    void* stream) {
+    if (!d_cube || !d_out || !d_range) return MI_ERR_ARGUMENT;
+    if (cube_bits != 32 && cube_bits != 64) return MI_ERR_ARGUMENT;
+    if (nx <= 0 || ny <= 0 || nc <= 0 || padx < 0 || pady < 0) return MI_ERR_ARGUMENT;
+    if (bitpix != -32 && bitpix != -64) return MI_ERR_ARGUMENT;
+    if (nx > 32767 || ny > 32767 || padx > 32767 || pady > 32767) return MI_ERR_LIMIT;
+    if (cube_to_fits(d_cube, cube_bits == 64, nx, ny, nc, padx, pady, bitpix == -64, d_out,
+                     d_range, (hipStream_t)stream) != hipSuccess)
+        return MI_ERR_DEVICE;
+    return MI_OK;
+}
+
 }  // extern "C"

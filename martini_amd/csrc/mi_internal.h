@@ -93,6 +93,8 @@ hipError_t prepare_particles(const double* xyz, const double* vxyz, const double
 
 // Beam convolution of a cube (mi_beam.hip).
 size_t beam_lds_bytes(int kx, int ky);
+hipError_t cube_to_fits(const void* cube, bool cube_f64, int nx, int ny, int nc, int padx,
+                        int pady, bool f64, void* out, double* range, hipStream_t s);
 hipError_t convolve_beam(const float* in, float* out, int nx, int ny, int nc, const float* K,
                          int kx, int ky, hipStream_t s);
 

@@ -124,7 +124,8 @@ class DataCube:
         x = crpix[0] - 1.0 + xi / cdelt[0]
         y = crpix[1] - 1.0 + eta / cdelt[1]
         v_ms = np.asarray(vlos_kms, float) * 1e3
-        spec = self._to_frequency(v_ms) if self._freq_channel_mode else v_ms
+        # radio convention; channel values are frequencies in FREQ mode
+        spec = HIfreq * (1.0 - v_ms / C_LIGHT) if self._freq_channel_mode else v_ms
         z = crpix[2] - 1.0 + (spec - crval[2]) / cdelt[2]
         return np.vstack((x, y, z))
 

@@ -13,8 +13,9 @@ path: records packed on the host (``_pack``), then binning, a tile gather
 on the device.  ``ncpu`` and ``progressbar`` are accepted for compatibility
 and have no effect.  A beam pads the cube at construction and
 ``convolve_beam`` convolves it on the GPU (martini.py:138-140, :863-901).
-Noise and the output writers are not part of this package: passing noise
-raises NotImplementedError.
+``write_fits`` / ``write_beam_fits`` write FITS files (martini.py:939-1152),
+the cube's data unit made on the GPU.  Noise and the HDF5 writer are not part
+of this package: passing noise raises NotImplementedError.
 """
 
 from __future__ import annotations
@@ -23,7 +24,7 @@ import warnings
 
 import numpy as np
 
-from . import _lib, _pack
+from . import _lib, _pack, fits
 from .insertion import default_inserter
 
 
@@ -184,6 +185,49 @@ class Martini(_BaseMartini):
                   f"  Maximum pixel: {a.max():.2e} Jy / beam",
                   f"  Median non-zero pixel: {np.median(a[a > 0]) if (a > 0).any() else 0.0:.2e}"
                   " Jy / beam", sep="\n")
+
+    def write_fits(self, filename: str, overwrite: bool = True, obj_name: str = "MOCK",
+                   channels=None, dtype=None) -> None:
+        """Write the cube to a FITS file (martini.py:939-1055): '.fits' is
+        appended unless present, the pad is dropped, the header holds the
+        reference's cards, and the data unit (FITS axis order, big-endian,
+        float64 unless ``dtype`` is float32) is made on the GPU."""
+        from .insertion import default_inserter
+
+        if channels is not None:
+            warnings.warn(DeprecationWarning(
+                "`channels` argument to `write_fits` ignored, channels and their units now "
+                "fixed at DataCube initialization."))
+        bitpix = fits.bitpix_for(dtype)
+        dc = self._datacube
+        dc.drop_pad()
+        filename = fits.fits_filename(filename)
+        ins = default_inserter()
+        torch = ins.torch
+        arr = dc._array[..., 0] if dc.stokes_axis else dc._array
+        nx, ny, nc = arr.shape
+        d_cube = torch.from_numpy(np.ascontiguousarray(arr, np.float64)).to(ins.device)
+        d_out = torch.empty(nx * ny * nc * (-bitpix // 8), dtype=torch.uint8, device=ins.device)
+        d_range = torch.empty(2, dtype=torch.float64, device=ins.device)
+        _lib.cube_to_fits(d_cube, nx, ny, nc, 0, 0, bitpix, d_out, d_range)
+        lo, hi = d_range.cpu().tolist()
+        data = d_out.cpu().numpy().tobytes()
+        cards = fits.cube_cards(dc, self.beam, obj_name, bitpix, lo, hi)
+        fits.write_hdu(filename, cards, data, overwrite)
+
+    def write_beam_fits(self, filename: str, overwrite: bool = True, channels=None) -> None:
+        """Write the beam image to a FITS file (martini.py:1057-1152)."""
+        if channels is not None:
+            warnings.warn(DeprecationWarning(
+                "`channels` argument to `write_fits` ignored, channels and their units now "
+                "fixed at DataCube initialization."))
+        if self.beam is None:
+            raise ValueError("Martini.write_beam_fits: Called with beam set to 'None'.")
+        filename = filename if filename[-5:] == ".fits" else filename + ".fits"
+        k = np.asarray(self.beam.kernel, np.float64)
+        # the reference writes kernel[..., newaxis].T: FITS axis 1 runs along kernel axis 0
+        data = np.ascontiguousarray(k[..., np.newaxis].T).astype(">f8").tobytes()
+        fits.write_hdu(filename, fits.beam_cards(self._datacube, self.beam, -64), data, True)
 
     def insert_source_in_cube(self, skip_validation: bool = False, progressbar=None,
                               ncpu: int = 1, method: str = "plain") -> None:

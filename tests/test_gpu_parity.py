@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from martini_amd import synthetic
+from oracle import cube_c
 from oracle import martini_oracle as O
 from tests.helpers import FLUX_TOL, VOXEL_TOL, gpu_cube, oracle_cube, oracle_kernel, parity
 
@@ -120,3 +121,32 @@ def test_global_profile(spectrum, method):
     print(f"global profile {spectrum} {method}: max {err:.3e} flux {flux:.3e}, kept {kept}")
     assert gp.source.npart == kept
     assert err <= VOXEL_TOL and flux <= FLUX_TOL
+
+
+@pytest.mark.parametrize("method", ["plain", "mfma"])
+def test_frequency_channels(method):
+    """A cube with channels evenly spaced in frequency (FREQ axis, radio
+    convention): the disc1e4 source, 32 channels of 19 kHz (about 4 km/s)."""
+    from martini_amd import DataCube, Martini, SPHSource
+    from martini_amd.sph_kernels import CubicSplineKernel
+    from martini_amd.spectral_models import GaussianSpectrum
+
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    a = w.arrays()
+    df = 19_000.0
+    cube = O.CubeSpec(w.n_px, w.n_px, w.n_channels, w.px_size, df, spectral_centre=w.vsys,
+                      freq_mode=True)
+    P = O.prepare(a.xyz, a.vxyz, a.mHI, a.hsm, a.T, distance_mpc=w.distance, cube=cube,
+                  kernel=oracle_kernel("cubic"), rotation=w.rotation.as_matrix())
+    ref = cube_c.cube(P) / w.px_size**2
+    src = SPHSource(distance=w.distance, rotation=w.rotation, T_g=a.T, mHI_g=a.mHI, xyz_g=a.xyz,
+                    vxyz_g=a.vxyz, hsm_g=a.hsm)
+    dc = DataCube(n_px_x=w.n_px, n_px_y=w.n_px, n_channels=w.n_channels, px_size=w.px_size,
+                  channel_width=df, spectral_centre=w.vsys, freq_channels=True)
+    m = Martini(source=src, datacube=dc, sph_kernel=CubicSplineKernel(),
+                spectral_model=GaussianSpectrum(sigma="thermal"), quiet=True)
+    m.insert_source_in_cube(method=method)
+    voxel, flux = parity(m.datacube._array, ref)
+    print(f"freq channels {method}: voxel {voxel:.3e} flux {flux:.3e}")
+    assert voxel <= VOXEL_TOL
+    assert flux <= FLUX_TOL

@@ -57,6 +57,26 @@ def test_setup_offset_pointing_and_padding():
     assert np.array_equal(m.sph_kernel.sm_ranges, P.sm_ranges)
 
 
+def test_frequency_mode_pixel_coordinates_match_oracle():
+    """FREQ cubes: line-of-sight velocities become frequencies (radio
+    convention) before the spectral pixel is formed."""
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    a = w.arrays(3000)
+    src = SPHSource(distance=5.0, rotation=w.rotation, ra=35.0, dec=-20.0, vpeculiar=30.0, T_g=a.T,
+                    mHI_g=a.mHI, xyz_g=a.xyz, vxyz_g=a.vxyz, hsm_g=a.hsm)
+    dc = DataCube(n_px_x=40, n_px_y=48, n_channels=24, px_size=20.0, channel_width=2.5e4,
+                  spectral_centre=380.0, ra=35.01, dec=-19.98, freq_channels=True)
+    m = Martini(source=src, datacube=dc, sph_kernel=K.WendlandC2Kernel(),
+                spectral_model=GaussianSpectrum(sigma="thermal"), quiet=True)
+    cube = O.CubeSpec(40, 48, 24, 20.0, 2.5e4, spectral_centre=380.0, ra_deg=35.01, dec_deg=-19.98,
+                      freq_mode=True)
+    P = O.prepare(a.xyz, a.vxyz, a.mHI, a.hsm, a.T, distance_mpc=5.0, cube=cube,
+                  kernel=O.adaptive(O.KernelSpec("wc2")), ra_deg=35.0, dec_deg=-20.0,
+                  vpec_kms=30.0, rotation=w.rotation.as_matrix())
+    assert m.source.npart == P.pixcoords.shape[1] > 0
+    np.testing.assert_allclose(m.source.pixcoords, P.pixcoords, atol=1e-8)
+
+
 def test_frequency_channels_match_oracle():
     dc = DataCube(n_px_x=8, n_px_y=8, n_channels=16, px_size=15.0, channel_width=18920.0,
                   spectral_centre=210.0, freq_channels=True)
