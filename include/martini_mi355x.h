@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 1
+#define MI_ABI_VERSION 2
 
 /* Error codes. */
 #define MI_OK 0
@@ -55,6 +55,7 @@ extern "C" {
  * DiracDeltaSpectrum :488. */
 #define MI_SPECTRUM_GAUSS 0
 #define MI_SPECTRUM_DIRAC 1
+#define MI_SPECTRUM_TABLE 2  /* spectra evaluated by the caller, read from a device table */
 
 /* Insertion methods. */
 #define MI_METHOD_PLAIN_F32 0   /* FP32 VALU tile gather */
@@ -76,7 +77,10 @@ extern "C" {
  * amp * 0.5*(erf(s*(u-c)) - erf(s*(u-c-1))) for a Gaussian line with
  * s = |de|/(sqrt(2) sigma), or amp for c in [c_lo, c_hi] for a Dirac line.
  * [c_lo, c_hi] is the window outside which the line is below 1e-10 of its
- * flux (Gaussian) or zero (Dirac).
+ * flux (Gaussian) or zero (Dirac).  With MI_SPECTRUM_TABLE (any spectral
+ * model's _BaseSpectrum.spectral_function, martini/spectral_models.py:
+ * 216-273, evaluated by the caller) channel c receives
+ * amp * spectra[k0 * spectra_ld + c] for c in [c_lo, c_hi]; fu and s are unused.
  */
 typedef struct mi_particle {
     int32_t ix, iy;
@@ -95,6 +99,9 @@ typedef struct mi_cube_desc {
     int32_t spectrum;        /* MI_SPECTRUM_* */
     float gauss_t[4];        /* Gaussian-kernel truncations by table index */
     float gauss_inv_norm[4]; /* 1 / _GaussianKernel.norm by table index */
+    const float* spectra;    /* MI_SPECTRUM_TABLE: device table of spectra, one row per
+                                record's k0 (null otherwise) */
+    int64_t spectra_ld;      /* its row stride in floats, >= nc */
 } mi_cube_desc;
 
 /* Library ABI version (MI_ABI_VERSION). */
@@ -224,7 +231,8 @@ int mi_prepare_workspace_size(int64_t n, size_t* bytes);
  * n_pairs that mi_insert_workspace_size and mi_insert take), MI_PREP_* flags,
  * and the pixels of the reference's square masks (martini/martini.py:271-274)
  * summed over kept particles (times nc: the particle-voxel updates).
- * Asynchronous.
+ * Asynchronous.  MI_ERR_UNSUPPORTED for MI_SPECTRUM_TABLE (the caller
+ * evaluates table spectra on the host).
  */
 int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
                const double* d_hsm, const double* d_T, const double* d_edges, int64_t n,

@@ -13,7 +13,7 @@ from .datacube import DataCube, HIfreq
 from .L_coords import L_coords
 from .martini import GlobalProfile, Martini
 from .sources import SPHSource
-from . import beams, sph_kernels, spectral_models
+from . import beams, noise, sph_kernels, spectral_models
 
-__all__ = ["Martini", "GlobalProfile", "DataCube", "SPHSource", "L_coords", "HIfreq", "beams", "sph_kernels",
-           "spectral_models"]
+__all__ = ["Martini", "GlobalProfile", "DataCube", "SPHSource", "L_coords", "HIfreq", "beams",
+           "noise", "sph_kernels", "spectral_models"]

@@ -18,7 +18,8 @@ MI_OK = 0
 MI_ERR_WORKSPACE = -2
 
 KERNEL_CUBIC, KERNEL_WC2, KERNEL_WC6, KERNEL_QUARTIC, KERNEL_GAUSS, KERNEL_DIRAC = range(6)
-SPECTRUM_GAUSS, SPECTRUM_DIRAC = 0, 1
+SPECTRUM_GAUSS, SPECTRUM_DIRAC, SPECTRUM_TABLE = 0, 1, 2
+ABI_VERSION = 2
 METHOD_PLAIN_F32, METHOD_MFMA_BF16X3 = 0, 1
 METHODS = {"plain": METHOD_PLAIN_F32, "fp32": METHOD_PLAIN_F32,
            "mfma": METHOD_MFMA_BF16X3, "bf16x3": METHOD_MFMA_BF16X3}
@@ -37,7 +38,8 @@ class CubeDesc(ctypes.Structure):
 
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nc", ctypes.c_int32),
                 ("spectrum", ctypes.c_int32), ("gauss_t", ctypes.c_float * 4),
-                ("gauss_inv_norm", ctypes.c_float * 4)]
+                ("gauss_inv_norm", ctypes.c_float * 4), ("spectra", ctypes.c_void_p),
+                ("spectra_ld", ctypes.c_int64)]
 
 
 class Observation(ctypes.Structure):
@@ -90,6 +92,9 @@ def lib() -> ctypes.CDLL:
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, f32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
     L.mi_abi_version.restype = ctypes.c_int
+    if L.mi_abi_version() != ABI_VERSION:
+        raise ImportError(f"{LIB_PATH} has ABI version {L.mi_abi_version()}, expected "
+                          f"{ABI_VERSION}: rebuild with python -m martini_amd.build_native")
     L.mi_strerror.restype = ctypes.c_char_p
     L.mi_strerror.argtypes = [ctypes.c_int]
     L.mi_count_pairs_host.restype = i64
@@ -148,9 +153,13 @@ def stream_ptr(torch_stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def make_desc(nx, ny, nc, spectrum, gauss_table=()) -> CubeDesc:
+def make_desc(nx, ny, nc, spectrum, gauss_table=(), spectra=None, spectra_ld=0) -> CubeDesc:
+    """Cube descriptor; ``spectra`` is the device table (a tensor or a
+    pointer) for SPECTRUM_TABLE, with row stride ``spectra_ld`` floats."""
     d = CubeDesc()
     d.nx, d.ny, d.nc, d.spectrum = int(nx), int(ny), int(nc), int(spectrum)
+    d.spectra = spectra.data_ptr() if hasattr(spectra, "data_ptr") else spectra
+    d.spectra_ld = int(spectra_ld)
     for k in range(4):
         d.gauss_t[k] = 0.0
         d.gauss_inv_norm[k] = 1.0

@@ -142,16 +142,43 @@ def make_records(n, ix, iy, fx, fy, h, r_kind, k0, fu, s, amp, c_lo, c_hi) -> np
     return rec
 
 
+def table_fields(spectral_model, source, datacube, out_scale=1.0):
+    """A spectral model without a device form (MI_SPECTRUM_TABLE): its spectra
+    evaluated on the host in FP64 (``_BaseSpectrum._host_spectra``) times the
+    output scale, as an FP32 N x Nc table; per particle k0 = its table row,
+    amp = 1, and [c_lo, c_hi] the first and last channel with a nonzero entry."""
+    S = spectral_model._host_spectra(source, datacube) * out_scale
+    n, nc = S.shape
+    nz = np.isfinite(S) & (S != 0.0)
+    S = np.where(nz, S, 0.0)
+    has = nz.any(axis=1)
+    c_lo = np.where(has, np.argmax(nz, axis=1), 1)
+    c_hi = np.where(has, nc - 1 - np.argmax(nz[:, ::-1], axis=1), 0)
+    z = np.zeros(n)
+    return (np.arange(n, dtype=np.int64), z, z, np.ones(n), c_lo.astype(np.int64),
+            c_hi.astype(np.int64), np.ascontiguousarray(S, np.float32))
+
+
 def pack_particles(source, sph_kernel, spectral_model, datacube, out_scale=1.0):
-    """Records and cube descriptor for an initialised (and pruned) source."""
+    """Records and cube descriptor for an initialised (and pruned) source.
+    For a table spectrum the descriptor carries the host table as
+    ``host_spectra``; the inserter uploads it and points the descriptor at it."""
     nx, ny, nc = datacube._array.shape[:3]
     ix, iy, fx, fy, h, r_kind, gauss = kernel_fields(sph_kernel, source.pixcoords, nx, ny)
     sc = source.skycoords
-    k0, fu, s, amp, c_lo, c_hi = spectral_fields(
-        sc.radial_velocity, spectral_model._sigma_kms(source), source.mHI_g, sc.distance,
-        datacube, spectral_model._mi_kind, out_scale)
+    table = None
+    if spectral_model._mi_kind == _lib.SPECTRUM_TABLE:
+        k0, fu, s, amp, c_lo, c_hi, table = table_fields(spectral_model, source, datacube,
+                                                         out_scale)
+    else:
+        k0, fu, s, amp, c_lo, c_hi = spectral_fields(
+            sc.radial_velocity, spectral_model._sigma_kms(source), source.mHI_g, sc.distance,
+            datacube, spectral_model._mi_kind, out_scale)
     rec = make_records(source.npart, ix, iy, fx, fy, h, r_kind, k0, fu, s, amp, c_lo, c_hi)
-    desc = _lib.make_desc(nx, ny, nc, spectral_model._mi_kind, gauss)
+    desc = _lib.make_desc(nx, ny, nc, spectral_model._mi_kind, gauss,
+                          spectra_ld=nc if table is not None else 0)
+    if table is not None:
+        desc.host_spectra = table
     return rec, desc
 
 

@@ -12,7 +12,10 @@ using namespace mi;
 static bool desc_ok(const mi_cube_desc* d) {
     if (!d) return false;
     if (d->nx <= 0 || d->ny <= 0 || d->nc <= 0) return false;
-    if (d->spectrum != MI_SPECTRUM_GAUSS && d->spectrum != MI_SPECTRUM_DIRAC) return false;
+    if (d->spectrum != MI_SPECTRUM_GAUSS && d->spectrum != MI_SPECTRUM_DIRAC &&
+        d->spectrum != MI_SPECTRUM_TABLE)
+        return false;
+    if (d->spectrum == MI_SPECTRUM_TABLE && (!d->spectra || d->spectra_ld < d->nc)) return false;
     return true;
 }
 
@@ -189,6 +192,8 @@ int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
                const double* d_hsm, const double* d_T, const double* d_edges, int64_t n,
                const mi_observation* obs, mi_particle* d_out, uint64_t* d_counts,
                void* d_workspace, size_t workspace_bytes, void* stream) {
+    // table spectra come from the caller's host evaluation, not from raw arrays
+    if (obs && obs->spectrum == MI_SPECTRUM_TABLE) return MI_ERR_UNSUPPORTED;
     if (!observation_ok(obs) || n < 0 || !d_counts) return MI_ERR_ARGUMENT;
     if (n > 0 && (!d_xyz || !d_vxyz || !d_mHI || !d_hsm || !d_out || !d_workspace))
         return MI_ERR_ARGUMENT;

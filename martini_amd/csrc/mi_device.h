@@ -328,8 +328,18 @@ __device__ inline float line_flux(int lo, int hi, int k0, float fu, float s, flo
     return 0.5f * amp * erf_span(s * (dd - 0.5f), s);
 }
 
-__device__ inline float channel_flux(const mi_particle& p, int c, int spectrum) {
-    return line_flux(p.c_lo, p.c_hi, p.k0, p.fu, p.s, p.amp, c, spectrum);
+// Spectrum value from the caller's table (MI_SPECTRUM_TABLE): amp times entry
+// c of table row `row`, inside the particle's window [lo, hi].
+__device__ inline float table_flux(int lo, int hi, int row, float amp, int c,
+                                   const float* __restrict__ tab, int64_t ld) {
+    if (c < lo || c > hi) return 0.0f;
+    return amp * tab[(int64_t)row * ld + c];
+}
+
+__device__ inline float channel_flux(const mi_particle& p, int c, const mi_cube_desc& d) {
+    if (d.spectrum == MI_SPECTRUM_TABLE)
+        return table_flux(p.c_lo, p.c_hi, p.k0, p.amp, c, d.spectra, d.spectra_ld);
+    return line_flux(p.c_lo, p.c_hi, p.k0, p.fu, p.s, p.amp, c, d.spectrum);
 }
 
 }  // namespace mi

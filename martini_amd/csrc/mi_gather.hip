@@ -97,7 +97,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             qm = (2u << ((j0 + (wk.kw >> 8) - 1) >> 2)) - (1u << (j0 >> 2));
         }
         fill_weights(cs, x0, y0, wtab, tkind, Tk, [&](int k, int px, float w) { Wt[k][px] = w; });
-        fill_spectra_rows(cs, nk, c0, g.nc, d.spectrum, [&](int k, int cl, float v) { St[k][cl] = v; });
+        fill_spectra_rows(cs, nk, c0, g.nc, d, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
         if (kmask == 0u) continue;
         while (kmask != 0u) {
@@ -242,13 +242,14 @@ __global__ void k_weights(int kind, float gt, float ginv, const float* __restric
     if (k < n) out[k] = pixel_weight(kind, (double)dx[k], (double)dy[k], h[k], gt, ginv, wtab);
 }
 
-__global__ void k_spectra(const mi_particle* __restrict__ P, int64_t n, int nc, int spectrum,
+__global__ void k_spectra(const mi_particle* __restrict__ P, int64_t n, mi_cube_desc d,
                           float* __restrict__ out) {
+    const int nc = d.nc;
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n * nc) return;
     int64_t p = e / nc;
     int c = (int)(e - p * nc);
-    out[e] = channel_flux(P[p], c, spectrum);
+    out[e] = channel_flux(P[p], c, d);
 }
 
 hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
@@ -288,8 +289,8 @@ hipError_t launch_spectra(const mi_particle* d_p, int64_t n, const mi_cube_desc&
                           hipStream_t s) {
     int64_t tot = n * d.nc;
     if (tot <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_spectra, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, d_p, n,
-                       d.nc, d.spectrum, out);
+    hipLaunchKernelGGL(k_spectra, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, d_p, n, d,
+                       out);
     return hipGetLastError();
 }
 

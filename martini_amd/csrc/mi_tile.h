@@ -220,8 +220,8 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
 // code (the window test is a select, the wide-channel erfc path one branch
 // per block) so that the evaluations of a thread overlap.
 template <class Store>
-__device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc, int spectrum,
-                                    Store store, uint32_t blocks = 0xFu) {
+__device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc,
+                                    const mi_cube_desc& d, Store store, uint32_t blocks = 0xFu) {
     const int k = threadIdx.x & (KC - 1);
     const int cl0 = threadIdx.x / KC;
     const mi_particle& p = cs.p[k];
@@ -229,8 +229,21 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     const int lo = live ? p.c_lo : 1, hi = live ? min(p.c_hi, nc - 1) : 0;
     const int k0 = p.k0;
     const float fu = p.fu, s = p.s, amp = p.amp;
+    const int spectrum = d.spectrum;
     const bool dirac = spectrum == MI_SPECTRUM_DIRAC;
     constexpr int PER_BLOCK = 16 / (NT / KC);  // a thread's channels in one 16-channel block
+    if (spectrum == MI_SPECTRUM_TABLE) {  // uniform: the caller's spectra
+#pragma unroll
+        for (int b = 0; b < CBLOCK / 16; ++b) {
+            if (!((blocks >> b) & 1u)) continue;
+#pragma unroll
+            for (int i = 0; i < PER_BLOCK; ++i) {
+                const int cl = cl0 + (NT / KC) * (b * PER_BLOCK + i);
+                store(k, cl, table_flux(lo, hi, k0, amp, c0 + cl, d.spectra, d.spectra_ld));
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int b = 0; b < CBLOCK / 16; ++b) {
         if (!((blocks >> b) & 1u)) continue;  // uniform
@@ -265,16 +278,13 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
 // bank conflicts, and the particle's fields are read once per wave.
 template <class Store>
 __device__ inline void fill_spectra_rows(const ChunkState& cs, int nk, int c0, int nc,
-                                         int spectrum, Store store) {
+                                         const mi_cube_desc& d, Store store) {
     const int cl = threadIdx.x & (CBLOCK - 1);
     const int c = c0 + cl;
 #pragma unroll 2
     for (int k = threadIdx.x / CBLOCK; k < KC; k += NT / CBLOCK) {
         float v = 0.0f;
-        if (k < nk && c < nc) {
-            const mi_particle& p = cs.p[k];
-            v = line_flux(p.c_lo, p.c_hi, p.k0, p.fu, p.s, p.amp, c, spectrum);
-        }
+        if (k < nk && c < nc) v = channel_flux(cs.p[k], c, d);
         store(k, cl, v);
     }
 }

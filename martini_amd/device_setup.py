@@ -74,6 +74,10 @@ def observation(*, distance: float, datacube, sph_kernel, spectral_model, vpecul
     o.de_ms = float((e[-1] - e[0]) / dc.n_channels)
     o.dv_max_kms = float(np.max(np.abs(np.diff(e))) / 1e3)
     kind = spectral_model._mi_kind
+    if kind == _lib.SPECTRUM_TABLE:
+        raise NotImplementedError(f"{type(spectral_model).__name__}: device setup covers the "
+                                  "Gaussian and Dirac lines; Martini(...) evaluates other "
+                                  "spectral models on the host")
     o.spectrum = kind
     if kind == _lib.SPECTRUM_GAUSS:
         mode = spectral_model.sigma_mode

@@ -53,11 +53,24 @@ class Inserter:
         """Insert host records; returns the device cube (float32, [nx][ny][nc])."""
         torch = self.torch
         n = int(records.size)
+        d_spectra = None
+        if desc.spectrum == _lib.SPECTRUM_TABLE:
+            host = getattr(desc, "host_spectra", None)
+            if host is None and not desc.spectra:
+                raise ValueError("a table spectrum needs its spectra (host_spectra or a device "
+                                 "pointer in the descriptor)")
+            if host is not None:
+                d_spectra = torch.from_numpy(np.ascontiguousarray(host, np.float32)).to(self.device)
+                desc.spectra = d_spectra.data_ptr()
+                desc.spectra_ld = host.shape[1]
         n_pairs = _lib.count_pairs(records, desc)
         d_rec = self.upload(records)
         if d_cube is None:
             d_cube = torch.zeros(desc.nx * desc.ny * desc.nc, dtype=torch.float32, device=self.device)
         self.insert_device(d_rec, n, n_pairs, desc, d_cube, method=method)
+        if d_spectra is not None:
+            torch.cuda.current_stream(self.device).synchronize()  # the table is read until here
+            desc.spectra = None
         return d_cube
 
 

@@ -13,9 +13,10 @@ path: records packed on the host (``_pack``), then binning, a tile gather
 on the device.  ``ncpu`` and ``progressbar`` are accepted for compatibility
 and have no effect.  A beam pads the cube at construction and
 ``convolve_beam`` convolves it on the GPU (martini.py:138-140, :863-901).
-``write_fits`` / ``write_beam_fits`` write FITS files (martini.py:939-1152),
-the cube's data unit made on the GPU.  Noise and the HDF5 writer are not part
-of this package: passing noise raises NotImplementedError.
+``add_noise`` adds a noise model's realisation (martini.py:903-937), drawn
+with numpy's generator as the reference draws it.  ``write_fits`` /
+``write_beam_fits`` write FITS files (martini.py:939-1152), the cube's data
+unit made on the GPU.  The HDF5 writer is not part of this package.
 """
 
 from __future__ import annotations
@@ -31,13 +32,11 @@ from .insertion import default_inserter
 class _BaseMartini:
     def __init__(self, *, source, datacube, beam=None, noise=None, sph_kernel, spectral_model,
                  quiet: bool = False, _prune_kwargs: dict | None = None) -> None:
-        if noise is not None:
-            raise NotImplementedError("noise is not part of martini_amd")
         self.quiet = quiet
         self.source = source
         self._datacube = datacube
         self.beam = beam
-        self.noise = None
+        self.noise = noise
         self.sph_kernel = sph_kernel
         self.spectral_model = spectral_model
         if self.beam is not None:
@@ -185,6 +184,31 @@ class Martini(_BaseMartini):
                   f"  Maximum pixel: {a.max():.2e} Jy / beam",
                   f"  Median non-zero pixel: {np.median(a[a > 0]) if (a > 0).any() else 0.0:.2e}"
                   " Jy / beam", sep="\n")
+
+    def add_noise(self) -> None:
+        """Add the noise model's realisation to the cube (martini.py:903-937):
+        Jy/beam converted with the beam area to Jy/arcsec^2, and on to Jy/pix^2
+        when the source is not inserted yet."""
+        if self.noise is None:
+            warnings.warn("Skipping noise, no noise object provided to Martini.")
+            return
+        if self.beam is None:
+            warnings.warn("Skipping noise, no beam object (required to estimate "
+                          "post-convolution noise level) provided to Martini.")
+            return
+        dc = self._datacube
+        noise = np.asarray(self.noise.generate(dc, self.beam), float) / self.beam.area
+        if dc._array_unit == "Jy/pix2":
+            noise = noise * dc.px_size**2
+        elif dc._array_unit != "Jy/arcsec2":
+            raise ValueError(f"cannot add noise to a cube in {dc._array_unit}: add it before "
+                             "the beam convolution")
+        dc._array = dc._array + noise
+        if not self.quiet:
+            print("Noise added.",
+                  f"  Noise cube RMS: {np.std(noise):.2e} (before beam convolution).",
+                  "  Data cube RMS after noise addition (before beam convolution): "
+                  f"{np.std(dc._array):.2e}", sep="\n")
 
     def write_fits(self, filename: str, overwrite: bool = True, obj_name: str = "MOCK",
                    channels=None, dtype=None) -> None:
