@@ -78,7 +78,7 @@ def time_gpu(rec, desc, n_pairs, method: str, steps: int, warmup: int):
 
     def step(ev=None):
         cube.zero_()
-        _lib.bin(d_rec, n, n_pairs, desc, ws, stream)
+        _lib.bin(d_rec, n, n_pairs, desc, ws, stream, method=method)
         if ev is not None:
             ev[0].record(stream)
         _lib.gather(d_rec, n, n_pairs, desc, cube, ws, method, stream)

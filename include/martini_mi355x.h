@@ -139,13 +139,18 @@ int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
  * The two stages of mi_insert, asynchronous (no host synchronisation), for
  * callers that time or overlap them.  mi_bin sorts the particles into cells
  * inside the workspace; mi_gather then evaluates the contraction over those
- * cells with `method` and adds it to d_cube.  The workspace must not be
- * reused between the two calls.  Overflow of n_pairs is reported by
- * mi_status (which synchronises the stream); a gather after an overflowed
- * bin leaves the cube untouched.
+ * cells with `method` and adds it to d_cube.  mi_bin's `method` names the
+ * gather it prepares for: the MFMA form gets each cell's particles in full
+ * Z order (its skips are per 32-particle chunk), the FP32 form a coarser,
+ * cheaper sort (its skips are per particle).  Either order gives a correct
+ * gather with either method.  The workspace must not be reused between the
+ * two calls.  Overflow of n_pairs is reported by mi_status (which
+ * synchronises the stream); a gather after an overflowed bin leaves the cube
+ * untouched.
  */
 int mi_bin(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
-           const mi_cube_desc* desc, void* d_workspace, size_t workspace_bytes, void* stream);
+           const mi_cube_desc* desc, void* d_workspace, size_t workspace_bytes, int method,
+           void* stream);
 int mi_gather(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
               const mi_cube_desc* desc, float* d_cube, void* d_workspace,
               size_t workspace_bytes, int method, void* stream);

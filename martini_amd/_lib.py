@@ -106,7 +106,8 @@ def lib() -> ctypes.CDLL:
     L.mi_insert.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, vp, ctypes.c_size_t,
                             ctypes.c_int, vp]
     L.mi_bin.restype = ctypes.c_int
-    L.mi_bin.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, ctypes.c_size_t, vp]
+    L.mi_bin.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, ctypes.c_size_t,
+                         ctypes.c_int, vp]
     L.mi_gather.restype = ctypes.c_int
     L.mi_gather.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, vp, ctypes.c_size_t,
                             ctypes.c_int, vp]
@@ -194,10 +195,13 @@ def insert(d_records, n: int, n_pairs: int, desc: CubeDesc, d_cube, d_ws, method
     check(rc, "mi_insert")
 
 
-def bin(d_records, n: int, n_pairs: int, desc: CubeDesc, d_ws, stream=None) -> None:
-    """Asynchronous binning stage (mi_bin)."""
+def bin(d_records, n: int, n_pairs: int, desc: CubeDesc, d_ws, stream=None,
+        method: str = "plain") -> None:
+    """Asynchronous binning stage (mi_bin), ordering the cells' particles for
+    the gather ``method`` that follows."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
     check(lib().mi_bin(ctypes.c_void_p(d_records.data_ptr()), n, n_pairs, ctypes.byref(desc),
-                       ctypes.c_void_p(d_ws.data_ptr()), d_ws.numel() * d_ws.element_size(),
+                       ctypes.c_void_p(d_ws.data_ptr()), d_ws.numel() * d_ws.element_size(), m,
                        stream_ptr(stream)), "mi_bin")
 
 

@@ -31,6 +31,14 @@ Grid make_grid(const mi_cube_desc& d) {
     return g;
 }
 
+// Lowest key bit the coarse sort orders on: the top 16 bits of the key
+// (cell, kernel group and the high bits of the Z order) fit two 8-bit radix
+// passes.  Within equal sorted bits the stable sort keeps emit order.
+static int coarse_begin_bit(const Grid& g) {
+    const int b = g.cell_bits + KEY_LOCAL_BITS - 16;
+    return b < 0 ? 0 : (b > 8 ? 8 : b);
+}
+
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(64) void k_build_items(const uint32_t* __restrict__
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
-                         char* ws, const Layout& L, hipStream_t s) {
+                         char* ws, const Layout& L, bool full_order, hipStream_t s) {
     uint32_t* counts = (uint32_t*)(ws + L.counts);
     uint32_t* offsets = (uint32_t*)(ws + L.offsets);
     uint32_t* keys = (uint32_t*)(ws + L.keys);
@@ -272,7 +280,8 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
                        vals, status);
     if (n_pairs > 0) {
         hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys2, vals, vals2,
-                                                 (size_t)n_pairs, 0,
+                                                 (size_t)n_pairs,
+                                                 full_order ? 0 : coarse_begin_bit(g),
                                                  g.cell_bits + KEY_LOCAL_BITS, s);
         if (e != hipSuccess) return e;
     }

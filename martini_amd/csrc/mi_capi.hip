@@ -80,13 +80,14 @@ static int prepare(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
 }
 
 int mi_bin(const mi_particle* d_particles, int64_t n, int64_t n_pairs, const mi_cube_desc* desc,
-           void* d_workspace, size_t workspace_bytes, void* stream) {
+           void* d_workspace, size_t workspace_bytes, int method, void* stream) {
+    if (method != MI_METHOD_PLAIN_F32 && method != MI_METHOD_MFMA_BF16X3) return MI_ERR_UNSUPPORTED;
     Grid g;
     Layout L;
     int rc = prepare(d_particles, n, n_pairs, desc, d_workspace, workspace_bytes, &g, &L);
     if (rc) return rc;
-    if (bin_particles(d_particles, n, n_pairs, g, (char*)d_workspace, L, (hipStream_t)stream) !=
-        hipSuccess)
+    if (bin_particles(d_particles, n, n_pairs, g, (char*)d_workspace, L,
+                      method == MI_METHOD_MFMA_BF16X3, (hipStream_t)stream) != hipSuccess)
         return MI_ERR_DEVICE;
     return MI_OK;
 }
@@ -136,7 +137,7 @@ int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
               int method, void* stream) {
     if (!d_cube) return MI_ERR_ARGUMENT;
     if (method != MI_METHOD_PLAIN_F32 && method != MI_METHOD_MFMA_BF16X3) return MI_ERR_UNSUPPORTED;
-    int rc = mi_bin(d_particles, n, n_pairs, desc, d_workspace, workspace_bytes, stream);
+    int rc = mi_bin(d_particles, n, n_pairs, desc, d_workspace, workspace_bytes, method, stream);
     if (rc) return rc;
     rc = mi_gather(d_particles, n, n_pairs, desc, d_cube, d_workspace, workspace_bytes, method,
                    stream);

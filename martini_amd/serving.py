@@ -360,7 +360,7 @@ class Service:
             ws = torch.empty(max(ws_bytes, 256), dtype=torch.uint8, device=self.dev)
             cube = torch.zeros(cube_n, dtype=torch.float32, device=self.dev)
             if kept > 0:
-                _lib.bin(rec, kept, pairs, d, ws, s)
+                _lib.bin(rec, kept, pairs, d, ws, s, method=self.method)
                 _lib.gather(rec, kept, pairs, d, cube, ws, self.method, s)
             host = self._pinned_cube(cube_n)
             host.copy_(cube, non_blocking=True)
