@@ -3,7 +3,7 @@
 // clipped to the cell's tile, and W evaluated once per (particle, footprint
 // pixel) pair.
 //
-// Staging is done by one wave with wave-wide scans (no serial loop); the
+// Staging is done by one wave with wave-wide DPP scans (no serial loop); the
 // waves take turns, and each loads its next chunk's records while earlier
 // chunks are processed, so the global-memory latency is hidden behind the
 // W/S/accumulate work.
@@ -42,6 +42,31 @@ struct ChunkState {
     uint32_t chanmask;  // 16-channel sub-blocks of the cell touched by any window
 };
 
+// Wave-wide inclusive scans with DPP (row shifts within 16-lane rows, then the
+// row broadcasts of lanes 15 and 31): VALU moves only, no LDS round trip as
+// with shuffles.  All 64 lanes must be active.
+template <class Op>
+__device__ inline uint32_t wave_scan(uint32_t x, Op op) {
+    // dpp_ctrl: row_shr:n = 0x110 + n, row_bcast:15 = 0x142, row_bcast:31 = 0x143;
+    // lanes whose source is out of range keep `old` (the identity 0 for + and |)
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false));
+    return x;
+}
+
+__device__ inline int wave_scan_add(int x) {
+    return (int)wave_scan((uint32_t)x, [](uint32_t a, uint32_t b) { return a + b; });
+}
+
+__device__ inline uint32_t wave_or(uint32_t x) {
+    return __builtin_amdgcn_readlane(wave_scan(x, [](uint32_t a, uint32_t b) { return a | b; }),
+                                     63);
+}
+
 // Called by all 64 lanes of one wave: lane t < KC holds particle t of the chunk
 // in `p` (valid when t < nk).  Clips footprints to the tile, scans the pair
 // counts, ORs the row and channel-block masks across the wave and forms each
@@ -67,27 +92,14 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
     } else {
         ilo = 1; ihi = 0; jlo = 1; jhi = 0;
     }
-    int incl = cnt;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        const int v = __shfl_up(incl, dd, 64);
-        if (lane >= dd) incl += v;
-    }
-#pragma unroll
-    for (int dd = 32; dd > 0; dd >>= 1) {
-        rows |= __shfl_xor(rows, dd, 64);
-        chans |= __shfl_xor(chans, dd, 64);
-    }
+    const int incl = wave_scan_add(cnt);
+    rows = wave_or(rows);
+    chans = wave_or(chans);
     // Each particle's pairs go to ceil(cnt / per) threads of their own, per =
     // ceil(total / (NT - KC)): at most total / per + KC <= NT threads in all.
-    const int total = __shfl(incl, 63, 64);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
     const int per = max(1, (total + (NT - KC) - 1) / (NT - KC));
-    int tincl = (cnt + per - 1) / per;
-#pragma unroll
-    for (int dd = 1; dd < 64; dd <<= 1) {
-        const int v = __shfl_up(tincl, dd, 64);
-        if (lane >= dd) tincl += v;
-    }
+    const int tincl = wave_scan_add((cnt + per - 1) / per);
     if (lane < KC) {
         if (valid) cs.p[lane] = p;
         cs.ilo[lane] = ilo;
