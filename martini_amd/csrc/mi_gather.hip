@@ -41,7 +41,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
 
     __shared__ __attribute__((aligned(16))) float Wt[KC][TILE_PX];
-    __shared__ __attribute__((aligned(16))) float St[KC][CBLOCK];
+    // rows padded to 68 floats: the S fill writes a column (32 particles) per
+    // store, which then spreads over the LDS banks
+    __shared__ __attribute__((aligned(16))) float St[KC][CBLOCK + 4];
     __shared__ ChunkState cs;
     __shared__ float4 Tk[WTAB_N];  // the item's kernel table
 
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             qm = (2u << ((j0 + (wk.kw >> 8) - 1) >> 2)) - (1u << (j0 >> 2));
         }
         fill_weights(cs, x0, y0, wtab, tkind, Tk, [&](int k, int px, float w) { Wt[k][px] = w; });
-        fill_spectra_rows(cs, nk, c0, g.nc, d, [&](int k, int cl, float v) { St[k][cl] = v; });
+        fill_spectra(cs, nk, c0, g.nc, d, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
         if (kmask == 0u) continue;
         while (kmask != 0u) {

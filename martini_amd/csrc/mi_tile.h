@@ -284,23 +284,6 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     }
 }
 
-// The same values as fill_spectra, with the lanes of a wave over the CBLOCK
-// channels of one particle (wave w takes particles w, w + 4, ...): a row of
-// an [particle][channel] tile is written by consecutive lanes, free of LDS
-// bank conflicts, and the particle's fields are read once per wave.
-template <class Store>
-__device__ inline void fill_spectra_rows(const ChunkState& cs, int nk, int c0, int nc,
-                                         const mi_cube_desc& d, Store store) {
-    const int cl = threadIdx.x & (CBLOCK - 1);
-    const int c = c0 + cl;
-#pragma unroll 2
-    for (int k = threadIdx.x / CBLOCK; k < KC; k += NT / CBLOCK) {
-        float v = 0.0f;
-        if (k < nk && c < nc) v = channel_flux(cs.p[k], c, d);
-        store(k, cl, v);
-    }
-}
-
 // Copies the table of the kernel kind of the item's first particle (when it
 // is a tabulated kind) into LDS; returns that kind, or -1.  Particles of one
 // cell are sorted with the tabulated kinds first, so this is the table the
