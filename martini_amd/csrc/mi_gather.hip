@@ -68,11 +68,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // stages (j + 4) and the sort index of the one after (j + 8): every global
     // load has four chunks' time to land.
     const int tkind = stage_table(P, order, it.begin, wtab, Tk);
-    mi_particle next;
+    RecRegs next{};
     uint32_t next_idx = 0;
     {
         const int b0 = it.begin + wave * KC;
-        if (lane < min(KC, it.end - b0)) next = P[order[b0 + lane]];
+        if (lane < min(KC, it.end - b0)) next = load_record(P, order[b0 + lane]);
         if (lane < min(KC, it.end - b0 - 4 * KC)) next_idx = order[b0 + 4 * KC + lane];
     }
     int chunk = 0;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         const int nk = min(KC, it.end - kb);
         if (wave == (chunk & 3)) {
             stage_chunk(next, lane < nk, x0, y0, c0, g, d, cs);
-            if (lane < min(KC, it.end - kb - 4 * KC)) next = P[next_idx];
+            if (lane < min(KC, it.end - kb - 4 * KC)) next = load_record(P, next_idx);
             if (lane < min(KC, it.end - kb - 8 * KC)) next_idx = order[kb + 8 * KC + lane];
         }
         __syncthreads();

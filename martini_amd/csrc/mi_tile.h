@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mi_device.h"
 #include "mi_internal.h"
 
@@ -67,13 +69,55 @@ __device__ inline uint32_t wave_or(uint32_t x) {
                                      63);
 }
 
+// A particle record held in registers between its prefetch and its staging,
+// as three 16-byte pieces.  (A prefetched mi_particle copied as a whole struct
+// can be given a per-thread LDS or scratch slot by the compiler, and the
+// store into that slot then waits for the load: the prefetch would stall.)
+struct RecRegs {
+    uint4 a, b, c;
+};
+
+__device__ inline RecRegs load_record(const mi_particle* __restrict__ P, uint32_t i) {
+    const uint4* q = reinterpret_cast<const uint4*>(P + i);
+    return RecRegs{q[0], q[1], q[2]};
+}
+
+__device__ inline mi_particle unpack_record(const RecRegs& r) {
+    mi_particle p;
+    p.ix = (int32_t)r.a.x;
+    p.iy = (int32_t)r.a.y;
+    p.fx = __uint_as_float(r.a.z);
+    p.fy = __uint_as_float(r.a.w);
+    p.h = __uint_as_float(r.b.x);
+    p.r_kind = (int32_t)r.b.y;
+    p.c_lo = (int32_t)r.b.z;
+    p.c_hi = (int32_t)r.b.w;
+    p.k0 = (int32_t)r.c.x;
+    p.fu = __uint_as_float(r.c.y);
+    p.s = __uint_as_float(r.c.z);
+    p.amp = __uint_as_float(r.c.w);
+    return p;
+}
+
+// Entry i (0..3) of a 4-entry kernel-argument array for a lane-varying i, by
+// selects over wave-uniform values: an indexed read of the kernel arguments
+// would be a memory load on the staging path.
+__device__ inline float pick4(const float (&v)[4], int i) {
+    const float v0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v[0])));
+    const float v1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v[1])));
+    const float v2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v[2])));
+    const float v3 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v[3])));
+    return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3;
+}
+
 // Called by all 64 lanes of one wave: lane t < KC holds particle t of the chunk
-// in `p` (valid when t < nk).  Clips footprints to the tile, scans the pair
+// in `r` (valid when t < nk).  Clips footprints to the tile, scans the pair
 // counts, ORs the row and channel-block masks across the wave and forms each
 // particle's weight factors.
-__device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int y0, int c0,
+__device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0, int c0,
                                    const Grid& g, const mi_cube_desc& d, ChunkState& cs) {
     const int lane = threadIdx.x & 63;
+    const mi_particle p = unpack_record(r);
     int cnt = 0, ilo = 1, ihi = 0, jlo = 1, jhi = 0;
     uint32_t rows = 0, chans = 0;
     if (valid && footprint(p, g.nx, g.ny, ilo, ihi, jlo, jhi)) {
@@ -110,7 +154,7 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
         cs.toff[lane + 1] = tincl;
         if (cnt > 0) {
             const int kind = kind_of(p.r_kind), gi = gidx_of(p.r_kind);
-            const WFactors f = weight_factors(kind, p.h, d.gauss_t[gi], d.gauss_inv_norm[gi]);
+            const WFactors f = weight_factors(kind, p.h, pick4(d.gauss_t, gi), pick4(d.gauss_inv_norm, gi));
             WalkRec w;
             w.px = (double)p.ix + (double)p.fx;
             w.py = (double)p.iy + (double)p.fy;
@@ -134,23 +178,26 @@ __device__ inline void stage_chunk(const mi_particle& p, bool valid, int x0, int
 }
 
 // The weight of a tabulated kernel at q^2 = dr^2 / h^2, as weight_at computes
-// it, without branches: fb = 1/h^2, tab the kernel's table, cubic whether the
-// seam term of the cubic spline applies.
-__device__ inline float table_weight_select(const float4* __restrict__ tab, float q2, float fb,
-                                            bool cubic) {
+// it, without branches: fb = 1/h^2, tab the kernel's table, CUBIC whether the
+// seam term of the cubic spline applies.  q is clamped to the table's range,
+// so the row is read for every pixel (no branch around the LDS load, and the
+// loads of neighbouring evaluations overlap), and the weight is masked to zero
+// for q >= 1 by the factor.
+template <bool CUBIC>
+__device__ inline float table_weight_select(const float4* __restrict__ tab, float q2, float fb) {
     const float q = __builtin_amdgcn_sqrtf(q2);
-    const float x = q * (float)WTAB_N;
+    const float x = fminf(q, 1.0f) * (float)WTAB_N;
     const int i = min((int)x, WTAB_N - 1);
     const float t = x - (float)i;
     const float4 c = tab[i];
-    float w = q < 1.0f ? fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x) : 0.0f;
-    if (cubic) {
+    float w = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+    if (CUBIC) {
         const float R2 = 4.0f * q2;
         const float seam =
             R2 * __builtin_amdgcn_sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
         w = q2 <= 0.25f ? w + seam : w;
     }
-    return w * fb;
+    return w * (q < 1.0f ? fb : 0.0f);
 }
 
 // Evaluates W for every (particle, pixel) pair of the chunk and hands each
@@ -182,25 +229,35 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
     const double px = w.px - (double)x0, py = w.py - (double)y0;   // particle in tile coordinates
 
     if (tab_kind >= 0 && __all(!active || kind == tab_kind)) {
-        const float fb = (float)w.fa;
-        const bool cubic = tab_kind == MI_KERNEL_CUBIC;
-        auto eval = [&](int ra, int rb) {
-            const double dx = px - (double)(ti + ra), dy = py - (double)(tj + rb);
-            return table_weight_select(tab, (float)((dx * dx + dy * dy) * w.fa), fb, cubic);
+        // The particle's offset from the footprint's first pixel in FP32 (one
+        // rounding of the exact FP64 value); q^2 follows in FP32, a few ulps
+        // from the FP64 form, far below the table's own error.
+        const float gx = (float)(px - (double)ti), gy = (float)(py - (double)tj);
+        const float fa = (float)w.fa;
+        auto walk = [&](auto cubic) {
+            auto eval = [&](int ra, int rb) {
+                const float dx = gx - (float)ra, dy = gy - (float)rb;
+                return table_weight_select<decltype(cubic)::value>(tab, fmaf(dx, dx, dy * dy) * fa,
+                                                                   fa);
+            };
+            int l = l0;
+            for (; l + 1 < l1; l += 2) {
+                const int a1 = a, b1 = b;
+                b = b + 1 == width ? 0 : b + 1;
+                a = b == 0 ? a + 1 : a;
+                const int a2 = a, b2 = b;
+                b = b + 1 == width ? 0 : b + 1;
+                a = b == 0 ? a + 1 : a;
+                const float v1 = eval(a1, b1), v2 = eval(a2, b2);
+                store(k, (ti + a1) * TILE + tj + b1, v1);
+                store(k, (ti + a2) * TILE + tj + b2, v2);
+            }
+            if (l < l1) store(k, (ti + a) * TILE + tj + b, eval(a, b));
         };
-        int l = l0;
-        for (; l + 1 < l1; l += 2) {
-            const int a1 = a, b1 = b;
-            b = b + 1 == width ? 0 : b + 1;
-            a = b == 0 ? a + 1 : a;
-            const int a2 = a, b2 = b;
-            b = b + 1 == width ? 0 : b + 1;
-            a = b == 0 ? a + 1 : a;
-            const float v1 = eval(a1, b1), v2 = eval(a2, b2);
-            store(k, (ti + a1) * TILE + tj + b1, v1);
-            store(k, (ti + a2) * TILE + tj + b2, v2);
-        }
-        if (l < l1) store(k, (ti + a) * TILE + tj + b, eval(a, b));
+        if (tab_kind == MI_KERNEL_CUBIC)  // uniform
+            walk(std::true_type{});
+        else
+            walk(std::false_type{});
         return;
     }
 
