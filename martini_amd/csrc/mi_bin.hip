@@ -32,7 +32,7 @@ Grid make_grid(const mi_cube_desc& d) {
 }
 
 // Lowest key bit the coarse sort orders on: the top 16 bits of the key
-// (cell, kernel group and the high bits of the Z order) fit two 8-bit radix
+// (cell, kernel group and the high bits of the index hash) fit two 8-bit radix
 // passes.  Within equal sorted bits the stable sort keeps emit order.
 static int coarse_begin_bit(const Grid& g) {
     const int b = g.cell_bits + KEY_LOCAL_BITS - 16;
@@ -101,13 +101,17 @@ __global__ void k_count(const mi_particle* __restrict__ P, int64_t n, Grid g,
 }
 
 // Writes the (key, particle) pairs of each particle at its scanned offset.
+// The low 8 key bits are the pixel's Z order in the tile, or, with `mix`, a
+// hash of the particle index: the FP32 gather's waves each own 4 tile rows, and
+// a chunk of particles drawn from all over the tile gives them even shares of
+// the products, where a spatially compact chunk loads one or two of them.
 // Entries past the device total up to the capacity get key 0xffffffff so the
 // sort moves them to the end; a total above the capacity raises the overflow
 // flag and writes nothing.
 __global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap, Grid g,
                        const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                       uint32_t* __restrict__ status) {
+                       uint32_t* __restrict__ status, int mix) {
     const int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     uint64_t total = n > 0 ? (uint64_t)offsets[n - 1] + counts[n - 1] : 0;
@@ -126,13 +130,15 @@ __global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap
         if (cells_of(p, g, tx0, tx1, ty0, ty1, cb0, cb1) == 0) continue;
         uint32_t at = offsets[k];
         uint32_t kg = kernel_group(kind_of(p.r_kind));
+        const uint32_t hash = ((uint32_t)k * 2654435761u) >> 24;
         for (int tx = tx0; tx <= tx1; ++tx) {
             int lx = p.ix - tx * TILE;
             lx = lx < 0 ? 0 : (lx > TILE - 1 ? TILE - 1 : lx);
             for (int ty = ty0; ty <= ty1; ++ty) {
                 int ly = p.iy - ty * TILE;
                 ly = ly < 0 ? 0 : (ly > TILE - 1 ? TILE - 1 : ly);
-                uint32_t local = (kg << 8) | spread4((uint32_t)lx) | (spread4((uint32_t)ly) << 1);
+                uint32_t local =
+                    (kg << 8) | (mix ? hash : spread4((uint32_t)lx) | (spread4((uint32_t)ly) << 1));
                 for (int cb = cb0; cb <= cb1; ++cb) {
                     uint32_t cell = (uint32_t)(((int64_t)tx * g.nty + ty) * g.ncb + cb);
                     keys[at] = (cell << KEY_LOCAL_BITS) | local;
@@ -277,7 +283,7 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
     unsigned eb = blocks_for(emit_threads, T);
     if (eb > 65536u) eb = 65536u;
     hipLaunchKernelGGL(k_emit, dim3(eb), dim3(T), 0, s, d_p, n, n_pairs, g, counts, offsets, keys,
-                       vals, status);
+                       vals, status, full_order ? 0 : 1);
     if (n_pairs > 0) {
         hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys2, vals, vals2,
                                                  (size_t)n_pairs,

@@ -61,8 +61,10 @@ enum { ST_PAIRS = 0, ST_OVERFLOW = 1, ST_ITEMS = 2, ST_MULTI = 3 };
 hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L);
 
 // Binning: count, scan, emit, sort, cell bounds, items.  Asynchronous.
-// full_order: sort on every key bit (full in-cell Z order, for the MFMA form);
-// otherwise on the top 16 (two radix passes) when the key is longer.
+// full_order: keys end in the pixel Z order and are sorted on every bit (the
+// MFMA form); otherwise they end in a hash of the particle index, which spreads
+// each chunk over its tile, and are sorted on the top 16 bits (two radix
+// passes) when the key is longer.
 hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
                          char* ws, const Layout& L, bool full_order, hipStream_t s);
 
