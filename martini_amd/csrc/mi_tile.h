@@ -285,9 +285,10 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
 // channels beyond the cube give 0.  Thread t takes particle t % KC and the
 // channels t / KC + 8 i, so its record fields are read once.  Only the
 // 16-channel blocks set in `blocks` (wave-uniform) are evaluated; the others
-// are left untouched.  The values are line_flux's, computed as straight-line
-// code (the window test is a select, the wide-channel erfc path one branch
-// per block) so that the evaluations of a thread overlap.
+// are left untouched.  The values are line_flux's (for narrow spans the same
+// series with its polynomial expanded, to FP32 rounding), computed as
+// straight-line code (the window test is a select, the wide-channel erfc path
+// one branch per block) so that the evaluations of a thread overlap.
 template <class Store>
 __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc,
                                     const mi_cube_desc& d, Store store, uint32_t blocks = 0xFu) {
@@ -313,6 +314,15 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
         }
         return;
     }
+    // Narrow spans: span_series with its D-dependent factors folded once per
+    // particle (D = s), 0.5 amp (erf(m + D/2) - erf(m - D/2)) =
+    // exp(-m^2) (e0 + e1 m^2 + e2 m^4 + e3 m^6); the same polynomial expanded.
+    const float D2 = s * s, K = amp * 0.564189583547756287f * s;  // amp D / sqrt(pi)
+    const float e0 = K * (1.0f + D2 * (-1.0f / 12.0f + D2 * (1.0f / 160.0f - D2 * (1.0f / 2688.0f))));
+    const float e1 = K * D2 * (1.0f / 6.0f + D2 * (-1.0f / 40.0f + D2 * (1.0f / 448.0f)));
+    const float e2 = K * D2 * D2 * (1.0f / 120.0f - D2 * (1.0f / 672.0f));
+    const float e3 = K * D2 * D2 * D2 * (1.0f / 5040.0f);
+    const float fum = fu - 0.5f;
 #pragma unroll
     for (int b = 0; b < CBLOCK / 16; ++b) {
         if (!((blocks >> b) & 1u)) continue;  // uniform
@@ -320,22 +330,22 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
 #pragma unroll
         for (int i = 0; i < PER_BLOCK; ++i) {
             const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK + i);
-            const float dd = (float)(k0 - c) + fu;
-            v[i] = span_series(s * (dd - 0.5f), s);
+            const float m = s * ((float)(k0 - c) + fum), m2 = m * m;
+            v[i] = __expf(-m2) * fmaf(fmaf(fmaf(e3, m2, e2), m2, e1), m2, e0);
         }
         if (!dirac && s > 0.5f) {
 #pragma unroll
             for (int i = 0; i < PER_BLOCK; ++i) {
                 const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK + i);
                 const float m = s * (((float)(k0 - c) + fu) - 0.5f);
-                v[i] = erf_diff(m - 0.5f * s, m + 0.5f * s);
+                v[i] = 0.5f * amp * erf_diff(m - 0.5f * s, m + 0.5f * s);
             }
         }
 #pragma unroll
         for (int i = 0; i < PER_BLOCK; ++i) {
             const int cl = cl0 + (NT / KC) * (b * PER_BLOCK + i);
             const int c = c0 + cl;
-            const float f = dirac ? amp : 0.5f * amp * v[i];
+            const float f = dirac ? amp : v[i];
             store(k, cl, (c >= lo && c <= hi) ? f : 0.0f);
         }
     }
