@@ -191,11 +191,9 @@ __device__ inline float table_weight_select(const float4* __restrict__ tab, floa
     const float t = x - (float)i;
     const float4 c = tab[i];
     float w = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-    if (CUBIC) {
-        const float R2 = 4.0f * q2;
-        const float seam =
-            R2 * __builtin_amdgcn_sqrtf(1.0f - R2) * (float)(1.0 / (8.0 * 1.59689476201133));
-        w = q2 <= 0.25f ? w + seam : w;
+    if (CUBIC) {  // R2 sqrt(1 - R2) / (8 * 1.5969), R2 = 4 q^2, zero for R2 >= 1
+        w = fmaf(q2 * (float)(4.0 / (8.0 * 1.59689476201133)),
+                 __builtin_amdgcn_sqrtf(fmaxf(fmaf(-4.0f, q2, 1.0f), 0.0f)), w);
     }
     return w * (q < 1.0f ? fb : 0.0f);
 }
@@ -322,15 +320,17 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     const float e1 = K * D2 * (1.0f / 6.0f + D2 * (-1.0f / 40.0f + D2 * (1.0f / 448.0f)));
     const float e2 = K * D2 * D2 * (1.0f / 120.0f - D2 * (1.0f / 672.0f));
     const float e3 = K * D2 * D2 * D2 * (1.0f / 5040.0f);
-    const float fum = fu - 0.5f;
+    // Channel c = c0 + cl0 + 8 j of this thread sits at k0 - c + fu - 1/2 =
+    // A - 8 j from the line centre (in channels), A formed once; subtracting
+    // the small integer 8 j from A is exact in the in-window range.
+    const float A = (float)(k0 - c0 - cl0) + (fu - 0.5f);
 #pragma unroll
     for (int b = 0; b < CBLOCK / 16; ++b) {
         if (!((blocks >> b) & 1u)) continue;  // uniform
         float v[PER_BLOCK];
 #pragma unroll
         for (int i = 0; i < PER_BLOCK; ++i) {
-            const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK + i);
-            const float m = s * ((float)(k0 - c) + fum), m2 = m * m;
+            const float m = s * (A - (float)((NT / KC) * (b * PER_BLOCK + i))), m2 = m * m;
             v[i] = __expf(-m2) * fmaf(fmaf(fmaf(e3, m2, e2), m2, e1), m2, e0);
         }
         if (!dirac && s > 0.5f) {
