@@ -35,7 +35,8 @@ __host__ __device__ inline int item_length(int64_t n_pairs) {
     int64_t len = (n_pairs / ITEM_TARGET + 31) / 32 * 32;
     return (int)(len < ITEM_MIN ? ITEM_MIN : (len > ITEM_MAX ? ITEM_MAX : len));
 }
-// Key layout: cell << KEY_LOCAL_BITS | kernel group << 8 | 8-bit pixel Morton code.
+// Key layout: cell << KEY_LOCAL_BITS | kernel group << 8 | 8-bit local order
+// (the pixel's Morton code in the tile, or a hash of the particle index).
 constexpr int KEY_LOCAL_BITS = 10;
 
 __host__ __device__ inline int kind_of(int32_t r_kind) { return r_kind & 15; }
