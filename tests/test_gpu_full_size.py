@@ -56,3 +56,25 @@ def test_full_size_properties(name):
 
     for method, first in (("plain", plain), ("mfma", mfma)):
         assert np.array_equal(_cube(rec, desc, method), first), method
+
+
+@pytest.mark.parametrize("method", ["plain", "mfma"])
+def test_order_independence(method):
+    """The cube does not depend on the order of the particle records beyond
+    FP32 summation order: a random permutation and a spatially sorted order
+    (the layout of many simulation snapshots) give the same cube within 2e-6 of
+    its peak and the same flux within 1e-7 (disc1e6_256px, 1e6 particles).  The
+    order only changes FP32 summation order (about 1e-6 of the peak here)."""
+    rec, desc = _records("disc1e6_256px")
+    base = _cube(rec, desc, method)
+    peak = np.abs(base).max()
+    rng = np.random.default_rng(23)
+    orders = {"random": rng.permutation(rec.size),
+              "spatial": np.lexsort((rec["iy"], rec["ix"]))}
+    for name, perm in orders.items():
+        other = _cube(rec[perm], desc, method)
+        voxel = np.abs(other.astype(np.float64) - base).max() / peak
+        flux = abs(other.sum(dtype=np.float64) - base.sum(dtype=np.float64)) / base.sum(
+            dtype=np.float64)
+        print(f"{method} {name} order: voxel {voxel:.3e} flux {flux:.3e}", flush=True)
+        assert voxel <= 2e-6 and flux <= 1e-7, name
