@@ -46,6 +46,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ __attribute__((aligned(16))) float St[KC][CBLOCK + 4];
     __shared__ ChunkState cs;
     __shared__ float4 Tk[WTAB_N];  // the item's kernel table
+    __shared__ GaussTables gt;     // the chunk's Gaussian-kernel factors
 
     const int t = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
@@ -98,7 +99,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             const int j0 = (wk.ij >> 16) - y0;
             qm = (2u << ((j0 + (wk.kw >> 8) - 1) >> 2)) - (1u << (j0 >> 2));
         }
-        fill_weights(cs, x0, y0, wtab, tkind, Tk, [&](int k, int px, float w) { Wt[k][px] = w; });
+        if (cs.gmask != 0u) {  // the same for every thread
+            fill_gauss_tables(cs, gt);
+            __syncthreads();
+        }
+        fill_weights(cs, x0, y0, wtab, tkind, Tk, gt, [&](int k, int px, float w) { Wt[k][px] = w; });
         fill_spectra(cs, nk, c0, g.nc, d, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
         if (kmask == 0u) continue;

@@ -76,6 +76,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ __attribute__((aligned(16))) char Bl[CBLOCK * KROW];   // S lo
     __shared__ ChunkState cs;
     __shared__ float4 Tk[WTAB_N];  // the item's kernel table
+    __shared__ GaussTables gt;     // the chunk's Gaussian-kernel factors
 
     const int t = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
@@ -118,7 +119,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         __syncthreads();
         const uint32_t rows = (__builtin_amdgcn_readfirstlane(cs.rowmask) >> (wave * 4)) & 15u;
         const uint32_t chans = __builtin_amdgcn_readfirstlane(cs.chanmask);
-        fill_weights(cs, x0, y0, wtab, tkind, Tk,
+        if (cs.gmask != 0u) {  // the same for every thread
+            fill_gauss_tables(cs, gt);
+            __syncthreads();
+        }
+        fill_weights(cs, x0, y0, wtab, tkind, Tk, gt,
                      [&](int k, int px, float w) { put_split(Ah, Al, px, k, w); });
         fill_spectra(cs, nk, c0, g.nc, d,
                      [&](int k, int cl, float s) { put_split(Bh, Bl, cl, k, s); }, chans);

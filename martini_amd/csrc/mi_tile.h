@@ -42,6 +42,17 @@ struct ChunkState {
     int per;            // pairs per thread
     uint32_t rowmask;   // tile rows (x offsets) touched by any footprint of the chunk
     uint32_t chanmask;  // 16-channel sub-blocks of the cell touched by any window
+    uint32_t gmask;     // particles of the Gaussian kernel with a footprint in the tile
+};
+
+// The separable factors of the Gaussian kernel's pixel weight for the chunk's
+// Gaussian particles, per footprint row and column: ex[k][r] = erf span of row
+// r of particle k's clipped footprint, ey[k][c] that of column c (weight_at's
+// ex and ey).  Filled once per chunk, they leave one erf and the FP64
+// truncation term per pixel.
+struct GaussTables {
+    float ex[KC][TILE];
+    float ey[KC][TILE];
 };
 
 // Wave-wide inclusive scans with DPP (row shifts within 16-lane rows, then the
@@ -139,6 +150,8 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
     const int incl = wave_scan_add(cnt);
     rows = wave_or(rows);
     chans = wave_or(chans);
+    const uint32_t gauss = (uint32_t)__ballot(lane < KC && cnt > 0 &&
+                                              kind_of(p.r_kind) == MI_KERNEL_GAUSS);
     // Each particle's pairs go to ceil(cnt / per) threads of their own, per =
     // ceil(total / (NT - KC)): at most total / per + KC <= NT threads in all.
     const int total = __builtin_amdgcn_readlane(incl, 63);
@@ -174,6 +187,7 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
         cs.per = per;
         cs.rowmask = rows;
         cs.chanmask = chans;
+        cs.gmask = gauss;
     }
 }
 
@@ -198,6 +212,35 @@ __device__ inline float table_weight_select(const float4* __restrict__ tab, floa
     return w * (q < 1.0f ? fb : 0.0f);
 }
 
+// Fills gt for the chunk's Gaussian particles (cs.gmask): thread t takes
+// particle t % KC and rows (then columns) t / KC, t / KC + NT / KC, ... of its
+// clipped footprint.  Loops end once no lane of the wave has a row (column)
+// left.  The values are weight_at's: the same offsets and spans.
+__device__ inline void fill_gauss_tables(const ChunkState& cs, GaussTables& gt) {
+    const int k = threadIdx.x & (KC - 1), slot = threadIdx.x / KC;
+    const bool gk = (cs.gmask >> k) & 1u;
+    const WalkRec w = cs.walk[k];
+    const float bf = cs.gf[k].x;
+    const int i0 = w.ij & 0xffff, j0 = w.ij >> 16;
+    const int rows = gk ? cs.ihi[k] - i0 + 1 : 0, cols = gk ? w.kw >> 8 : 0;
+    for (int r = slot; __any(r < rows); r += NT / KC)
+        if (r < rows) gt.ex[k][r] = erf_span((float)(w.px - (double)(i0 + r)) * bf, bf);
+    for (int c = slot; __any(c < cols); c += NT / KC)
+        if (c < cols) gt.ey[k][c] = erf_span((float)(w.py - (double)(j0 + c)) * bf, bf);
+}
+
+// The Gaussian kernel's weight as weight_at computes it, its separable
+// factors ex, ey read from the tables: zero beyond the truncation edge, and
+// the truncation term zmax = sqrt(t^2 - u^2) formed in FP64.
+__device__ inline float gauss_weight(double dx, double dy, double fa, double fe, float t,
+                                     float n, float ex, float ey) {
+    const double dr2 = dx * dx + dy * dy;
+    if (dr2 > fe) return 0.0f;
+    const double z2 = (double)t * (double)t - dr2 * fa;
+    const float zmax = z2 > 0.0 ? __builtin_amdgcn_sqrtf((float)z2) : 0.0f;
+    return 0.25f * erff(zmax * 0.70710678118654752f) * ex * ey * n;
+}
+
 // Evaluates W for every (particle, pixel) pair of the chunk and hands each
 // value to store(k, pixel-in-tile, w).  Particle k's pairs (pixel row l / width,
 // column l % width of its clipped footprint, l < cnt) are split over threads
@@ -208,7 +251,8 @@ __device__ inline float table_weight_select(const float4* __restrict__ tab, floa
 template <class Store>
 __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                                     const float4* __restrict__ wtab, int tab_kind,
-                                    const float4* __restrict__ tab, Store store) {
+                                    const float4* __restrict__ tab, const GaussTables& gt,
+                                    Store store) {
     const int t = threadIdx.x;
     const bool active = t < cs.toff[KC];
     int k = 0;
@@ -259,14 +303,23 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
         return;
     }
 
-    WFactors f{w.fa, 0.0, (float)w.fa, 0.0f, 0.0f};
-    if (kind == MI_KERNEL_GAUSS) {
+    if (kind == MI_KERNEL_GAUSS || !active) {
+        // Gaussian particles: the separable factors come from the chunk's tables
+        const double fe = cs.ge[k];
         const float4 gg = cs.gf[k];
-        f.e = cs.ge[k];
-        f.b = gg.x;
-        f.t = gg.y;
-        f.n = gg.z;
+        for (int l = l0; l < l1; ++l) {
+            const int i = x0 + ti + a, j = y0 + tj + b;
+            store(k, (ti + a) * TILE + tj + b,
+                  gauss_weight(w.px - (double)i, w.py - (double)j, w.fa, fe, gg.y, gg.z,
+                               gt.ex[k][a], gt.ey[k][b]));
+            if (++b == width) {
+                b = 0;
+                ++a;
+            }
+        }
+        return;
     }
+    const WFactors f{w.fa, 0.0, (float)w.fa, 0.0f, 0.0f};
     for (int l = l0; l < l1; ++l) {
         const int i = x0 + ti + a, j = y0 + tj + b;
         store(k, (ti + a) * TILE + tj + b,
