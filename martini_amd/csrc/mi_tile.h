@@ -387,9 +387,12 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
             v[i] = __expf(-m2) * fmaf(fmaf(fmaf(e3, m2, e2), m2, e1), m2, e0);
         }
         if (!dirac && s > 0.5f) {
+            // wide channels: a line this narrow spans a few channels, and the
+            // erfc differences are taken only inside its window
 #pragma unroll
             for (int i = 0; i < PER_BLOCK; ++i) {
                 const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK + i);
+                if (c < lo || c > hi) continue;
                 const float m = s * (((float)(k0 - c) + fu) - 0.5f);
                 v[i] = 0.5f * amp * erf_diff(m - 0.5f * s, m + 0.5f * s);
             }
