@@ -286,9 +286,13 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
     hipLaunchKernelGGL(k_emit, dim3(eb), dim3(T), 0, s, d_p, n, n_pairs, g, counts, offsets, keys,
                        vals, status, full_order ? 0 : 1);
     if (n_pairs > 0) {
+        // The MFMA form wants compact chunks: it takes the two-pass sort too when
+        // that still orders at least the top 5 Z-order bits (2 x 4-pixel blocks,
+        // on grids of at most 2^9 cells), and all key bits otherwise.
+        const int cb = coarse_begin_bit(g);
+        const int begin_bit = full_order ? (cb <= 3 ? cb : 0) : cb;
         hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys2, vals, vals2,
-                                                 (size_t)n_pairs,
-                                                 full_order ? 0 : coarse_begin_bit(g),
+                                                 (size_t)n_pairs, begin_bit,
                                                  g.cell_bits + KEY_LOCAL_BITS, s);
         if (e != hipSuccess) return e;
     }
