@@ -64,32 +64,42 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) acc[jj][cc] = 0.0f;
 
-    // Chunk j of the item is staged by wave j % 4, so the staging work is spread
-    // over the SIMDs.  Each wave prefetches the record of the next chunk it
-    // stages (j + 4) and the sort index of the one after (j + 8): every global
-    // load has four chunks' time to land.
+    // Every wave holds the records of the next chunk and the sort indices of the
+    // one after.  A chunk is staged by the wave that had the fewest particles to
+    // multiply in the chunk before (it reaches the staging first); every wave
+    // picks it from the same counts.
     const int tkind = stage_table(P, order, it.begin, wtab, Tk);
     RecRegs next{};
     uint32_t next_idx = 0;
-    {
-        const int b0 = it.begin + wave * KC;
-        if (lane < min(KC, it.end - b0)) next = load_record(P, order[b0 + lane]);
-        if (lane < min(KC, it.end - b0 - 4 * KC)) next_idx = order[b0 + 4 * KC + lane];
-    }
-    int chunk = 0;
+    if (lane < min(KC, it.end - it.begin)) next = load_record(P, order[it.begin + lane]);
+    if (lane < min(KC, it.end - it.begin - KC)) next_idx = order[it.begin + KC + lane];
+    int stager = 0;
 
-    for (int kb = it.begin; kb < it.end; kb += KC, ++chunk) {
+    for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
-        if (wave == (chunk & 3)) {
+        if (wave == __builtin_amdgcn_readfirstlane(stager))
             stage_chunk(next, lane < nk, x0, y0, c0, g, d, cs);
-            if (lane < min(KC, it.end - kb - 4 * KC)) next = load_record(P, next_idx);
-            if (lane < min(KC, it.end - kb - 8 * KC)) next_idx = order[kb + 8 * KC + lane];
-        }
+        if (lane < min(KC, it.end - kb - KC)) next = load_record(P, next_idx);
+        if (lane < min(KC, it.end - kb - 2 * KC)) next_idx = order[kb + 2 * KC + lane];
         __syncthreads();
-        // Particles whose footprint meets this wave's 4 rows (wave-uniform).
+        // Particles whose footprint meets each wave's 4 rows (wave-uniform masks);
+        // this wave keeps its own, and the least loaded wave stages the next chunk.
         const int klo = cs.ilo[lane & (KC - 1)], khi = cs.ihi[lane & (KC - 1)];
-        const bool hit = lane < KC && klo <= khi && khi >= wrow0 && klo <= wrow1;
-        uint32_t kmask = (uint32_t)__ballot(hit);
+        const bool valid = lane < KC && klo <= khi;
+        uint32_t kmask = 0;
+        int fewest = KC + 1;
+#pragma unroll
+        for (int w2 = 0; w2 < NT / 64; ++w2) {
+            const int r0 = x0 + w2 * 4;
+            const uint32_t m = (uint32_t)__ballot(valid && khi >= r0 && klo <= r0 + 3);
+            const int c = __builtin_popcount(m);
+            if (c < fewest) {
+                fewest = c;
+                stager = w2;
+            }
+            if (w2 == wave) kmask = m;
+        }
+        const bool hit = lane < KC && ((kmask >> lane) & 1u);
         // Lane k: the 4-column groups of the tile that particle k's footprint
         // covers (W is zero in the others).  Read here, before the next chunk's
         // staging may overwrite the chunk state.
