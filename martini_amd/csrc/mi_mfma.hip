@@ -95,30 +95,35 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) acc[r][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    // Chunk j of the item is staged by wave j % 4, so the staging work is spread
-    // over the SIMDs.  Each wave prefetches the record of the next chunk it
-    // stages (j + 4) and the sort index of the one after (j + 8): every global
-    // load has four chunks' time to land.
+    // Every wave holds the records of the next chunk and the sort indices of the
+    // one after.  A chunk is staged by the wave that had the fewest tile rows to
+    // multiply in the chunk before (it reaches the staging first); every wave
+    // picks it from the same row mask.
     const int tkind = stage_table(P, order, it.begin, wtab, Tk);
     RecRegs next{};
     uint32_t next_idx = 0;
-    {
-        const int b0 = it.begin + wave * KC;
-        if (lane < min(KC, it.end - b0)) next = load_record(P, order[b0 + lane]);
-        if (lane < min(KC, it.end - b0 - 4 * KC)) next_idx = order[b0 + 4 * KC + lane];
-    }
-    int chunk = 0;
+    if (lane < min(KC, it.end - it.begin)) next = load_record(P, order[it.begin + lane]);
+    if (lane < min(KC, it.end - it.begin - KC)) next_idx = order[it.begin + KC + lane];
+    int stager = 0;
 
-    for (int kb = it.begin; kb < it.end; kb += KC, ++chunk) {
+    for (int kb = it.begin; kb < it.end; kb += KC) {
         const int nk = min(KC, it.end - kb);
-        if (wave == (chunk & 3)) {
-            stage_chunk(next, lane < nk, x0, y0, c0, g, d, cs);
-            if (lane < min(KC, it.end - kb - 4 * KC)) next = load_record(P, next_idx);
-            if (lane < min(KC, it.end - kb - 8 * KC)) next_idx = order[kb + 8 * KC + lane];
-        }
+        if (wave == stager) stage_chunk(next, lane < nk, x0, y0, c0, g, d, cs);
+        if (lane < min(KC, it.end - kb - KC)) next = load_record(P, next_idx);
+        if (lane < min(KC, it.end - kb - 2 * KC)) next_idx = order[kb + 2 * KC + lane];
         __syncthreads();
-        const uint32_t rows = (__builtin_amdgcn_readfirstlane(cs.rowmask) >> (wave * 4)) & 15u;
+        const uint32_t rowmask = __builtin_amdgcn_readfirstlane(cs.rowmask);
+        const uint32_t rows = (rowmask >> (wave * 4)) & 15u;
         const uint32_t chans = __builtin_amdgcn_readfirstlane(cs.chanmask);
+        int fewest = 5;
+#pragma unroll
+        for (int w2 = 0; w2 < NT / 64; ++w2) {
+            const int c = __builtin_popcount((rowmask >> (w2 * 4)) & 15u);
+            if (c < fewest) {
+                fewest = c;
+                stager = w2;
+            }
+        }
         if (cs.gmask != 0u) {  // the same for every thread
             fill_gauss_tables(cs, gt);
             __syncthreads();
