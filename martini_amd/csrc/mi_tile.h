@@ -3,10 +3,11 @@
 // clipped to the cell's tile, and W evaluated once per (particle, footprint
 // pixel) pair.
 //
-// Staging is done by one wave with wave-wide DPP scans (no serial loop); the
-// waves take turns, and each loads its next chunk's records while earlier
-// chunks are processed, so the global-memory latency is hidden behind the
-// W/S/accumulate work.
+// Staging is done by one wave with wave-wide DPP scans (no serial loop): the
+// gathers pick the wave with the least product work in the previous chunk.
+// Every wave loads the next chunk's records while the current one is
+// processed, so the global-memory latency is hidden behind the W/S/accumulate
+// work.
 #pragma once
 
 #include <hip/hip_runtime.h>
