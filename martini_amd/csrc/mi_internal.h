@@ -52,7 +52,8 @@ struct Layout {
     size_t temp_bytes;
     size_t total;
     int64_t items_max, multi_max;
-    int item_len;                      // particles per work item
+    int item_len;                      // particles per work item (MFMA form)
+    int item_len_f32;                  // particles per work item (FP32 form)
 };
 
 // status[0]: number of pairs the device produced; status[1]: overflow flag.
