@@ -52,8 +52,8 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     };
     size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)(n_pairs > 0 ? n_pairs : 1);
     size_t nc1 = (size_t)g.ncells + 1;
-    L->item_len = item_length(n_pairs, ITEM_TARGET_MFMA);
-    L->item_len_f32 = item_length(n_pairs, ITEM_TARGET_F32);
+    L->item_len = item_length(n_pairs, ITEM_MAX_MFMA);
+    L->item_len_f32 = item_length(n_pairs, ITEM_MAX_F32);
     const int shortest = L->item_len < L->item_len_f32 ? L->item_len : L->item_len_f32;
     L->items_max = g.ncells + n_pairs / shortest + 1;
     L->multi_max = 2 * (n_pairs / shortest) + 1;

@@ -24,18 +24,18 @@ constexpr int TILE_PX = TILE * TILE;
 constexpr int CBLOCK = 64;
 // Particles per cell processed by one work item: longer cells are split and
 // their partial tiles summed in item order by a reduction pass.  The split
-// length depends only on the pair count and the form (about `target` items,
-// between ITEM_MIN and ITEM_MAX particles each), so a given input always sums
-// in the same order.  The FP32 form balances best with about 4096 items, the
-// MFMA form (whose items finish faster) with about 2048.
+// length depends only on the pair count and the form (about ITEM_TARGET
+// items, between ITEM_MIN and `longest` particles each), so a given input
+// always sums in the same order.  The FP32 form, whose items take longer,
+// balances best with items of at most 2048 particles, the MFMA form with 4096.
 constexpr int ITEM_MIN = 256;
-constexpr int ITEM_MAX = 4096;
-constexpr int ITEM_TARGET_MFMA = 2048;
-constexpr int ITEM_TARGET_F32 = 4096;
+constexpr int ITEM_TARGET = 2048;
+constexpr int ITEM_MAX_MFMA = 4096;
+constexpr int ITEM_MAX_F32 = 2048;
 
-__host__ __device__ inline int item_length(int64_t n_pairs, int target) {
-    int64_t len = (n_pairs / target + 31) / 32 * 32;
-    return (int)(len < ITEM_MIN ? ITEM_MIN : (len > ITEM_MAX ? ITEM_MAX : len));
+__host__ __device__ inline int item_length(int64_t n_pairs, int longest) {
+    int64_t len = (n_pairs / ITEM_TARGET + 31) / 32 * 32;
+    return (int)(len < ITEM_MIN ? ITEM_MIN : (len > longest ? longest : len));
 }
 // Key layout: cell << KEY_LOCAL_BITS | kernel group << 8 | 8-bit local order
 // (the pixel's Morton code in the tile, or a hash of the particle index).
