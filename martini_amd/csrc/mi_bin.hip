@@ -52,11 +52,9 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     };
     size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)(n_pairs > 0 ? n_pairs : 1);
     size_t nc1 = (size_t)g.ncells + 1;
-    L->item_len = item_length(n_pairs, ITEM_MAX_MFMA);
-    L->item_len_f32 = item_length(n_pairs, ITEM_MAX_F32);
-    const int shortest = L->item_len < L->item_len_f32 ? L->item_len : L->item_len_f32;
-    L->items_max = g.ncells + n_pairs / shortest + 1;
-    L->multi_max = 2 * (n_pairs / shortest) + 1;
+    L->item_len = item_length(n_pairs);
+    L->items_max = g.ncells + n_pairs / L->item_len + 1;
+    L->multi_max = 2 * (n_pairs / L->item_len) + 1;
     L->counts = take(4 * nn);
     L->offsets = take(4 * nn);
     L->keys = take(4 * np);
@@ -303,9 +301,9 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
         hipLaunchKernelGGL(k_cell_bounds, dim3(blocks_for(n_pairs, T)), dim3(T), 0, s, keys2,
                            n_pairs, g.ncells, cstart, cend);
     hipLaunchKernelGGL(k_item_scan, dim3(1), dim3(SCAN_T), 0, s, cstart, cend, g.ncells,
-                       full_order ? L.item_len : L.item_len_f32, status, ioff, moff);
+                       L.item_len, status, ioff, moff);
     hipLaunchKernelGGL(k_build_items, dim3((unsigned)g.ncells), dim3(64), 0, s, cstart, cend,
-                       full_order ? L.item_len : L.item_len_f32, ioff, moff, items);
+                       L.item_len, ioff, moff, items);
     return hipGetLastError();
 }
 

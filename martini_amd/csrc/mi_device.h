@@ -24,18 +24,16 @@ constexpr int TILE_PX = TILE * TILE;
 constexpr int CBLOCK = 64;
 // Particles per cell processed by one work item: longer cells are split and
 // their partial tiles summed in item order by a reduction pass.  The split
-// length depends only on the pair count and the form (about ITEM_TARGET
-// items, between ITEM_MIN and `longest` particles each), so a given input
-// always sums in the same order.  The FP32 form, whose items take longer,
-// balances best with items of at most 2048 particles, the MFMA form with 4096.
+// length depends only on the pair count (about ITEM_TARGET items, between
+// ITEM_MIN and ITEM_MAX particles each), so a given input always sums in the
+// same order.
 constexpr int ITEM_MIN = 256;
+constexpr int ITEM_MAX = 2048;
 constexpr int ITEM_TARGET = 2048;
-constexpr int ITEM_MAX_MFMA = 4096;
-constexpr int ITEM_MAX_F32 = 2048;
 
-__host__ __device__ inline int item_length(int64_t n_pairs, int longest) {
+__host__ __device__ inline int item_length(int64_t n_pairs) {
     int64_t len = (n_pairs / ITEM_TARGET + 31) / 32 * 32;
-    return (int)(len < ITEM_MIN ? ITEM_MIN : (len > longest ? longest : len));
+    return (int)(len < ITEM_MIN ? ITEM_MIN : (len > ITEM_MAX ? ITEM_MAX : len));
 }
 // Key layout: cell << KEY_LOCAL_BITS | kernel group << 8 | 8-bit local order
 // (the pixel's Morton code in the tile, or a hash of the particle index).

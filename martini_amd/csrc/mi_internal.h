@@ -52,8 +52,7 @@ struct Layout {
     size_t temp_bytes;
     size_t total;
     int64_t items_max, multi_max;
-    int item_len;                      // particles per work item (MFMA form)
-    int item_len_f32;                  // particles per work item (FP32 form)
+    int item_len;                      // particles per work item
 };
 
 // status[0]: number of pairs the device produced; status[1]: overflow flag.
