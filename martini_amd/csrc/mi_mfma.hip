@@ -20,9 +20,11 @@
 // the row index: the 16 rows a 16-lane group reads for one k-group then fall
 // in 16 distinct 16-byte bank slots.
 //
-// Per chunk there are two barriers: staging -> (W, S fill) -> products.  Each
-// wave clears the W rows it used right after reading them, so the next fill
-// finds them zero; S is fully rewritten by every fill.
+// Per chunk there are two barriers: staging -> (W, S fill) -> products, and a
+// third in chunks with Gaussian-kernel particles, whose row and column factors
+// are tabulated before the W fill.  Each wave clears the W rows it used right
+// after reading them, so the next fill finds them zero; every fill rewrites
+// the S blocks that the products will read.
 // Accumulation order is fixed: deterministic, no atomics.
 #include <hip/hip_runtime.h>
 
