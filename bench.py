@@ -71,7 +71,10 @@ def time_gpu(rec, desc, n_pairs, method: str, steps: int, warmup: int):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     n = int(rec.size)
-    d_rec = torch.from_numpy(np.ascontiguousarray(rec).view(np.uint8)).to(dev)
+    from martini_amd.insertion import shuffle_records
+
+    # the records' HBM layout of the insertion path (Inserter.upload)
+    d_rec = shuffle_records(torch.from_numpy(np.ascontiguousarray(rec).view(np.uint8)).to(dev))
     cube = torch.zeros(desc.nx * desc.ny * desc.nc, dtype=torch.float32, device=dev)
     ws = torch.empty(_lib.workspace_bytes(desc, n, n_pairs), dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()

@@ -132,7 +132,11 @@ class DeviceSource:
 
 def prepare(src: DeviceSource, obs, datacube=None, stream=None, d_edges=None, out=None, ws=None):
     """Runs mi_prepare; returns (d_records, d_counts) with d_counts a 4-element
-    int64 device tensor (kept, cell pairs, flags, mask pixels).  Asynchronous."""
+    int64 device tensor (kept, cell pairs, flags, mask pixels).  Asynchronous.
+    The kept records are in input order; for the FP32 form, inputs that come
+    spatially ordered are best inserted from
+    ``insertion.shuffle_records(d_records[: kept * 48])``, the order host
+    records take at upload."""
     import torch
 
     dev = src.device

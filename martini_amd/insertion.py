@@ -37,9 +37,10 @@ class Inserter:
         return self._ws
 
     def upload(self, records: np.ndarray):
-        """Copy host records into HBM (uint8 tensor of n * 48 bytes)."""
+        """Copy host records into HBM (uint8 tensor of n * 48 bytes), in the
+        order of ``shuffle_records``."""
         rec = np.ascontiguousarray(records)
-        return self.torch.from_numpy(rec.view(np.uint8)).to(self.device)
+        return shuffle_records(self.torch.from_numpy(rec.view(np.uint8)).to(self.device))
 
     def insert_device(self, d_records, n: int, n_pairs: int, desc, d_cube, method="plain",
                       stream=None) -> None:
@@ -72,6 +73,34 @@ class Inserter:
             torch.cuda.current_stream(self.device).synchronize()  # the table is read until here
             desc.spectra = None
         return d_cube
+
+
+SHUFFLE_SEED = 0x6D617274
+
+
+def shuffle_records(d_rec):
+    """Records (uint8 tensor of n * 48 bytes) in a fixed pseudo-random order,
+    drawn and gathered on the records' device (``torch.randperm`` with a
+    generator seeded by SHUFFLE_SEED, so the same n gives the same order).
+
+    The FP32 gather's coarse sort keeps a cell's particles in record order
+    within each bucket of its index hash, and takes them 32 at a time.
+    Records that arrive spatially ordered (as ordered snapshots do) would give
+    spatially compact chunks, which load one or two of the gather's four
+    waves; shuffled records give chunks spread over the tile whatever the input
+    order.  A structured reordering (a golden-ratio or other Weyl sweep) would
+    interact with the multiplicative index hash.  The inserted cube depends on
+    the order only through the FP32 summation order."""
+    import torch
+
+    n = d_rec.numel() // _lib.PARTICLE_DTYPE.itemsize
+    if n <= 1:
+        return d_rec
+    gen = torch.Generator(device=d_rec.device)
+    gen.manual_seed(SHUFFLE_SEED)
+    idx = torch.randperm(n, generator=gen, device=d_rec.device)
+    words = d_rec.view(torch.int32).view(n, _lib.PARTICLE_DTYPE.itemsize // 4)
+    return words.index_select(0, idx).view(-1).view(torch.uint8)
 
 
 _default = {}
