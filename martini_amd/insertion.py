@@ -90,12 +90,22 @@ def shuffle_records(d_rec):
     waves; shuffled records give chunks spread over the tile whatever the input
     order.  A structured reordering (a golden-ratio or other Weyl sweep) would
     interact with the multiplicative index hash.  The inserted cube depends on
-    the order only through the FP32 summation order."""
+    the order only through the FP32 summation order.
+
+    The shuffle needs, for a moment, a second copy of the records and the
+    permutation with its sort space (taken as 32 bytes per record); when the
+    device has not that much memory free (with a 1 GiB margin), the records
+    keep the given order."""
     import torch
 
     n = d_rec.numel() // _lib.PARTICLE_DTYPE.itemsize
     if n <= 1:
         return d_rec
+    if d_rec.device.type == "cuda":
+        free, _ = torch.cuda.mem_get_info(d_rec.device)
+        free += torch.cuda.memory_reserved(d_rec.device) - torch.cuda.memory_allocated(d_rec.device)
+        if free < n * (_lib.PARTICLE_DTYPE.itemsize + 32) + (1 << 30):
+            return d_rec
     gen = torch.Generator(device=d_rec.device)
     gen.manual_seed(SHUFFLE_SEED)
     idx = torch.randperm(n, generator=gen, device=d_rec.device)
