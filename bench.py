@@ -5,11 +5,17 @@ One step = one insertion of a synthetic disc into its cube, with the particle
 records already resident in HBM: zero the cube, bin and sort the particles
 into cells (mi_bin), gather and reduce (mi_gather).  The metric is
 particle-voxel updates per second, U = Nc * sum_p n_x(p) n_y(p) over the
-reference's square mask (martini/martini.py:271-274).  The reference's own
-CPU algorithm (oracle/cpu_baseline.py) is timed beside it on the host cores.
+reference's square mask (martini/martini.py:271-274).
+
+The default workload is the 10^7-particle disc on a 512^2 x 256 cube, where
+the BF16x3 MFMA form is the headline and the FP32 form is timed in the same
+run; the 10^6-particle disc on 256^2 x 128 follows as a secondary line.  The
+reference's own CPU algorithm (oracle/cpu_baseline.py) is timed beside them on
+the host cores: a sample of the headline workload with all host threads, and
+the 10^4-particle disc in full on one thread.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--method plain|mfma] [--no-cpu-baseline] [--serving N]
+                    [--method both|plain|mfma] [--no-cpu-baseline] [--serving N]
 
 The serving leg (``--serving N`` requests, default 512) runs the
 continuous-batching service of martini_amd.serving on a mixed request stream
@@ -32,6 +38,9 @@ sys.path.insert(0, ROOT)
 PEAK_TFLOPS = {"plain": 157.3, "mfma": 2500.0}  # MI355X dense FP32 / BF16 (public figures)
 PRODUCTS = {"plain": 1, "mfma": 3}  # products per logical multiply-add (bf16x3 spends 3)
 DTYPE = {"plain": "fp32", "mfma": "bf16x3"}
+BOUND = {"plain": "valu_fp32", "mfma": "mfma"}
+GATHER_KERNEL = {"plain": "k_gather_plain", "mfma": "k_gather_mfma"}
+TRAFFIC_FILE = "profiles/hbm_traffic_gather.md"
 METRIC = "particle-voxel updates/s; p50/p99 cube latency; MFMA vs FP32 at 1e-5 rel err"
 
 
@@ -50,51 +59,70 @@ def updates_of(m) -> int:
 
 
 def prepare(workload_name: str):
+    """Host FP64 setup (Martini's constructor) and record packing, timed."""
     from martini_amd import Martini, _lib, _pack, synthetic
 
     w = synthetic.WORKLOADS[workload_name]
     src, dc, k, s = synthetic.build(w)
+    t0 = time.perf_counter()
     m = Martini(source=src, datacube=dc, sph_kernel=k, spectral_model=s, quiet=True)
+    t1 = time.perf_counter()
     rec, desc = _pack.pack_particles(m.source, m.sph_kernel, m.spectral_model, dc,
                                      out_scale=1.0 / dc.px_size**2)
     n_pairs = _lib.count_pairs(rec, desc)
-    return w, m, rec, desc, n_pairs
+    t2 = time.perf_counter()
+    host = {"setup_s": t1 - t0, "pack_s": t2 - t1}
+    return w, m, rec, desc, n_pairs, host
 
 
-def time_gpu(rec, desc, n_pairs, method: str, steps: int, warmup: int):
-    """ms per step, ms of the gather stage (HIP events on the launch stream),
-    and binning status."""
+def upload(rec, method: str):
+    """Records into HBM as the insertion path lays them out (Inserter.upload);
+    returns the device records and the host-to-device copy time in ms."""
+    import torch
+
+    from martini_amd.insertion import layout_records
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    host = torch.from_numpy(np.ascontiguousarray(rec).view(np.uint8))
+    d = torch.empty(host.shape, dtype=torch.uint8, device=dev)  # allocation outside the copy
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.copy_(host)
+    torch.cuda.synchronize()
+    h2d_ms = (time.perf_counter() - t0) * 1e3
+    return layout_records(d, method), h2d_ms
+
+
+def time_gpu(d_rec, n, desc, n_pairs, method: str, steps: int, warmup: int):
+    """ms per step, ms of the gather stage and of binning (HIP events on the
+    launch stream), and binning status."""
     import torch
 
     from martini_amd import _lib
 
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    n = int(rec.size)
-    from martini_amd.insertion import shuffle_records
-
-    # the records' HBM layout of the insertion path (Inserter.upload)
-    d_rec = shuffle_records(torch.from_numpy(np.ascontiguousarray(rec).view(np.uint8)).to(dev))
+    dev = d_rec.device
     cube = torch.zeros(desc.nx * desc.ny * desc.nc, dtype=torch.float32, device=dev)
     ws = torch.empty(_lib.workspace_bytes(desc, n, n_pairs), dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
 
     def step(ev=None):
         cube.zero_()
-        _lib.bin(d_rec, n, n_pairs, desc, ws, stream, method=method)
         if ev is not None:
             ev[0].record(stream)
-        _lib.gather(d_rec, n, n_pairs, desc, cube, ws, method, stream)
+        _lib.bin(d_rec, n, n_pairs, desc, ws, stream, method=method)
         if ev is not None:
             ev[1].record(stream)
+        _lib.gather(d_rec, n, n_pairs, desc, cube, ws, method, stream)
+        if ev is not None:
+            ev[2].record(stream)
 
     for _ in range(warmup):
         step()
     st = _lib.status(desc, n, n_pairs, ws, stream)
     if st[1]:
         raise RuntimeError("pair capacity overflow")
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(steps)]
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -104,14 +132,15 @@ def time_gpu(rec, desc, n_pairs, method: str, steps: int, warmup: int):
     t1.record(stream)
     torch.cuda.synchronize()
     ms = t0.elapsed_time(t1) / steps
-    gather_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-    return ms, gather_ms, st, cube
+    bin_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in evs]))
+    gather_ms = float(np.mean([b.elapsed_time(c) for _, b, c in evs]))
+    return ms, gather_ms, bin_ms, st, cube
 
 
 def measured_traffic(workload: str, method: str):
     """HBM bytes per gather-stage launch from the committed FETCH_SIZE /
     WRITE_SIZE counter runs (profiles/hbm_traffic_gather.md), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_gather.md")
+    path = os.path.join(ROOT, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
     in_stage = False
@@ -208,15 +237,99 @@ def serving_leg(n_requests: int, method: str, slots: int, load: float, seed: int
     }
 
 
+def time_forms(workload: str, methods, steps: int, warmup: int) -> dict:
+    """Host setup, upload and the timed steps of each method on one workload."""
+    import torch
+
+    t = time.time()
+    w, m, rec, desc, n_pairs, host = prepare(workload)
+    U = updates_of(m)
+    n = int(rec.size)
+    log(f"[bench] {workload}: {n} particles, {n_pairs} cell pairs, U={U:.3e}, "
+        f"host setup {host['setup_s']:.1f}s + packing {host['pack_s']:.1f}s "
+        f"({time.time() - t:.1f}s with the disc)")
+    forms = {}
+    for method in methods:
+        d_rec, h2d_ms = upload(rec, method)
+        ms, gather_ms, bin_ms, st, _ = time_gpu(d_rec, n, desc, n_pairs, method, steps, warmup)
+        del d_rec
+        torch.cuda.empty_cache()
+        flops = 2.0 * U * PRODUCTS[method]
+        achieved = flops / (gather_ms * 1e-3) / 1e12
+        peak = PEAK_TFLOPS[method]
+        traffic = measured_traffic(workload, method)
+        forms[method] = {
+            "ms_per_step": ms,
+            "bin_ms": bin_ms,
+            "gather_ms": gather_ms,
+            "updates_per_s": U / (ms * 1e-3),
+            "dtype": DTYPE[method],
+            "work_items": int(st[2]),
+            "split_items": int(st[3]),
+            "h2d_ms": h2d_ms,
+            "pcie_inclusive_updates_per_s": U / ((ms + h2d_ms) * 1e-3),
+            "roofline": {
+                "bound": BOUND[method], "kernel": GATHER_KERNEL[method] + " + k_reduce",
+                "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": traffic,
+                "traffic_source": (f"{TRAFFIC_FILE} (rocprofv3 FETCH_SIZE / WRITE_SIZE runs of "
+                                   "this workload and method)") if traffic is not None else None,
+            },
+        }
+        log(f"[bench] {workload} {method}: {ms:.3f} ms/step (bin {bin_ms:.3f}, gather "
+            f"{gather_ms:.3f}), {U / (ms * 1e-3):.3e} updates/s, roofline "
+            f"{achieved:.1f}/{peak:.0f} TFLOP/s, items {st[2]}, split items {st[3]}, "
+            f"H2D {h2d_ms:.1f} ms")
+    return {"workload": w, "martini": m, "desc": desc, "U": U, "n": n, "n_pairs": n_pairs,
+            "host": host, "forms": forms}
+
+
+def cpu_legs(headline, budget_s: float) -> dict:
+    """The reference's algorithm on the host: a sample of the headline
+    workload with every host thread, and the 10^4-particle disc in full on one
+    thread (its configuration's ncpu = 1)."""
+    from martini_amd import synthetic
+    from oracle import cpu_baseline
+    from oracle import martini_oracle as O
+
+    def prepared(w):
+        a = w.arrays()
+        cube = O.CubeSpec(w.n_px, w.n_px, w.n_channels, w.px_size, w.channel_width,
+                          spectral_centre=w.vsys)
+        K = O.adaptive(O.KernelSpec(w.kernel))
+        return O.prepare(a.xyz, a.vxyz, a.mHI, a.hsm, a.T, distance_mpc=w.distance, cube=cube,
+                         kernel=K, rotation=w.rotation.as_matrix())
+
+    w = headline["workload"]
+    log(f"[bench] cpu baseline: {w.name}, sampling ~{budget_s:.0f}s of the reference loop")
+    main = cpu_baseline.measure(prepared(w), budget_s=budget_s)
+    main.pop("seconds_extrapolated", None)
+    main["workload"] = w.name
+    main["host_cpus"] = os.cpu_count()
+    legs = [dict(main)]
+    w1 = synthetic.WORKLOADS["disc1e4_64px"]
+    log(f"[bench] cpu baseline: {w1.name} in full, ncpu=1")
+    c1 = cpu_baseline.run_full(prepared(w1), ncpu=1)
+    c1["workload"] = w1.name
+    legs.append(c1)
+    main["legs"] = legs
+    log(f"[bench] cpu baseline: {main['value']:.3e} updates/s ({main['cores']} threads, "
+        f"{w.name}); {c1['value']:.3e} updates/s ({w1.name}, 1 thread)")
+    return main
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="disc1e6_256px")
-    ap.add_argument("--method", default="plain", choices=["plain", "mfma"])
+    ap.add_argument("--workload", default="disc1e7_512px")
+    ap.add_argument("--method", default="both", choices=["both", "plain", "mfma"],
+                    help="'both': the MFMA form is the headline, the FP32 form is timed beside it")
+    ap.add_argument("--secondary", default="disc1e6_256px",
+                    help="workload of the secondary line, both forms ('' skips it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="one device copy of this many MiB before timing (counter calibration)")
     ap.add_argument("--beam-fwhm-px", type=float, default=3.0,
@@ -233,11 +346,6 @@ def main(argv=None) -> int:
               file=sys.stderr)
         return 2
 
-    t = time.time()
-    w, m, rec, desc, n_pairs = prepare(args.workload)
-    U = updates_of(m)
-    log(f"[bench] {args.workload}: {rec.size} particles, {n_pairs} cell pairs, U={U:.3e}, "
-        f"host setup {time.time() - t:.1f}s")
     if args.copy_probe_mb > 0:
         import torch
 
@@ -246,15 +354,20 @@ def main(argv=None) -> int:
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    ms, gather_ms, st, _ = time_gpu(rec, desc, n_pairs, args.method, args.steps, args.warmup)
-    log(f"[bench] {args.method}: {ms:.3f} ms/step, gather {gather_ms:.3f} ms, items {st[2]}, "
-        f"split items {st[3]}")
-    value = U / (ms * 1e-3)
-    flops = 2.0 * U * PRODUCTS[args.method]
-    achieved = flops / (gather_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS[args.method]
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": measured_traffic(args.workload, args.method)}
+    methods = ["mfma", "plain"] if args.method == "both" else [args.method]
+    head = time_forms(args.workload, methods, args.steps, args.warmup)
+    lead = methods[0]
+    f = head["forms"][lead]
+    w, m, desc, U = head["workload"], head["martini"], head["desc"], head["U"]
+    roofline = dict(f["roofline"])
+
+    secondary = None
+    if args.secondary and args.secondary != args.workload:
+        sec = time_forms(args.secondary, ["mfma", "plain"], args.steps, args.warmup)
+        secondary = {"workload": args.secondary, "particles": sec["n"],
+                     "updates_per_step": sec["U"], "cell_pairs": sec["n_pairs"],
+                     "host": sec["host"], "forms": sec["forms"]}
+        del sec
 
     beam = beam_leg(desc, args.beam_fwhm_px) if args.beam_fwhm_px > 0 else None
     if beam is not None:
@@ -265,49 +378,44 @@ def main(argv=None) -> int:
     if args.serving > 0:
         srv = serving_leg(args.serving, args.serving_method, args.serving_slots, args.serving_load)
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import cpu_baseline
-        from oracle import martini_oracle as O
+    cpu = None if args.no_cpu_baseline else cpu_legs(head, args.cpu_seconds)
 
-        a = w.arrays()
-        cube = O.CubeSpec(w.n_px, w.n_px, w.n_channels, w.px_size, w.channel_width,
-                          spectral_centre=w.vsys)
-        K = O.adaptive(O.KernelSpec(w.kernel))
-        P = O.prepare(a.xyz, a.vxyz, a.mHI, a.hsm, a.T, distance_mpc=w.distance, cube=cube,
-                      kernel=K, rotation=w.rotation.as_matrix())
-        log(f"[bench] cpu baseline: sampling ~{args.cpu_seconds:.0f}s of the reference loop")
-        cpu = cpu_baseline.measure(P, budget_s=args.cpu_seconds)
-        cpu.pop("seconds_extrapolated", None)
-
+    forms = head["forms"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": f["updates_per_s"],
         "unit": "particle-voxel updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": f["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPE[args.method],
+        "dtype": DTYPE[lead],
         "data": "synthetic",
         "config": {
             "workload": args.workload,
-            "particles": int(rec.size),
+            "particles": head["n"],
             "cube": f"{w.n_px}x{w.n_px}x{w.n_channels}",
             "px_arcsec": w.px_size,
             "channel_kms": w.channel_width,
             "kernel": type(m.sph_kernel).__name__,
             "spectrum": "GaussianSpectrum(thermal)",
-            "method": args.method,
+            "method": lead,
             "updates_per_step": U,
-            "cell_pairs": n_pairs,
-            "gather_ms": gather_ms,
+            "cell_pairs": head["n_pairs"],
+            "bin_ms": f["bin_ms"],
+            "gather_ms": f["gather_ms"],
         },
+        "forms": forms,
+        "mfma_over_plain": (forms["plain"]["ms_per_step"] / forms["mfma"]["ms_per_step"]
+                            if "plain" in forms and "mfma" in forms else None),
+        "host": dict(head["host"], h2d_ms=f["h2d_ms"],
+                     pcie_inclusive_updates_per_s=f["pcie_inclusive_updates_per_s"]),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "secondary": secondary,
         "serving": srv,
         "beam_convolution": beam,
     }

@@ -36,11 +36,11 @@ class Inserter:
             self._ws = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def upload(self, records: np.ndarray):
-        """Copy host records into HBM (uint8 tensor of n * 48 bytes), in the
-        order of ``shuffle_records``."""
+    def upload(self, records: np.ndarray, method="plain"):
+        """Copy host records into HBM (uint8 tensor of n * 48 bytes), laid out
+        for `method` (``layout_records``)."""
         rec = np.ascontiguousarray(records)
-        return shuffle_records(self.torch.from_numpy(rec.view(np.uint8)).to(self.device))
+        return layout_records(self.torch.from_numpy(rec.view(np.uint8)).to(self.device), method)
 
     def insert_device(self, d_records, n: int, n_pairs: int, desc, d_cube, method="plain",
                       stream=None) -> None:
@@ -65,7 +65,7 @@ class Inserter:
                 desc.spectra = d_spectra.data_ptr()
                 desc.spectra_ld = host.shape[1]
         n_pairs = _lib.count_pairs(records, desc)
-        d_rec = self.upload(records)
+        d_rec = self.upload(records, method)
         if d_cube is None:
             d_cube = torch.zeros(desc.nx * desc.ny * desc.nc, dtype=torch.float32, device=self.device)
         self.insert_device(d_rec, n, n_pairs, desc, d_cube, method=method)
@@ -75,13 +75,35 @@ class Inserter:
         return d_cube
 
 
-SHUFFLE_SEED = 0x6D617274
+def _mix32(h):
+    """murmur3's 32-bit finaliser on an int64 tensor of values in [0, 2^32):
+    a bijection of 32-bit words, formed without leaving int64 range (each
+    product is split into 16-bit halves of the constant)."""
+    M = 0xFFFFFFFF
+
+    def mul(x, c):
+        return (x * (c & 0xFFFF) + (((x * (c >> 16)) & 0xFFFF) << 16)) & M
+
+    h = h ^ (h >> 16)
+    h = mul(h, 0x85EBCA6B)
+    h = h ^ (h >> 13)
+    h = mul(h, 0xC2B2AE35)
+    return h ^ (h >> 16)
+
+
+def record_permutation(n: int, device):
+    """The fixed pseudo-random order of n records: the indices sorted by a
+    bijective 32-bit hash of themselves.  The keys are distinct, so the order
+    is the same on every device and on the CPU."""
+    import torch
+
+    k = torch.arange(n, dtype=torch.int64, device=device)
+    return torch.argsort(_mix32(_mix32(k) ^ 0x6D617274))
 
 
 def shuffle_records(d_rec):
-    """Records (uint8 tensor of n * 48 bytes) in a fixed pseudo-random order,
-    drawn and gathered on the records' device (``torch.randperm`` with a
-    generator seeded by SHUFFLE_SEED, so the same n gives the same order).
+    """Records (uint8 tensor of n * 48 bytes) in the fixed pseudo-random order
+    of ``record_permutation``, gathered on the records' device.
 
     The FP32 gather's coarse sort keeps a cell's particles in record order
     within each bucket of its index hash, and takes them 32 at a time.
@@ -106,11 +128,19 @@ def shuffle_records(d_rec):
         free += torch.cuda.memory_reserved(d_rec.device) - torch.cuda.memory_allocated(d_rec.device)
         if free < n * (_lib.PARTICLE_DTYPE.itemsize + 32) + (1 << 30):
             return d_rec
-    gen = torch.Generator(device=d_rec.device)
-    gen.manual_seed(SHUFFLE_SEED)
-    idx = torch.randperm(n, generator=gen, device=d_rec.device)
+    idx = record_permutation(n, d_rec.device)
     words = d_rec.view(torch.int32).view(n, _lib.PARTICLE_DTYPE.itemsize // 4)
     return words.index_select(0, idx).view(-1).view(torch.uint8)
+
+
+def layout_records(d_rec, method="plain"):
+    """The records' HBM layout for a gather method: shuffled for the FP32 form
+    (``shuffle_records``), as given for the MFMA form, whose binning orders each
+    cell's particles fully (pixel Z order) so the record order does not reach
+    its chunks."""
+    if _lib.METHODS[method] == _lib.METHOD_PLAIN_F32:
+        return shuffle_records(d_rec)
+    return d_rec
 
 
 _default = {}

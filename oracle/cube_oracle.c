@@ -186,6 +186,12 @@ int oracle_cube(int64_t n, const double* x, const double* y, const double* h_eff
         int64_t p0 = n * t / nt, p1 = n * (t + 1) / nt;
         for (int64_t p = p0; p < p1 && S; ++p) {
             spectrum_row(v[p], sigma[p], amp[p], edges, nc, dirac_spectrum, S);
+            /* channels outside [c0, c1] hold exact zeros (erf saturates in FP64), and
+             * adding w * 0 leaves a voxel unchanged: they are skipped */
+            int c0 = 0, c1 = nc - 1;
+            while (c0 <= c1 && S[c0] == 0.0) ++c0;
+            while (c1 >= c0 && S[c1] == 0.0) --c1;
+            if (c0 > c1) continue;
             int ilo = (int)ceil(x[p] - r[p]), ihi = (int)floor(x[p] + r[p]);
             int jlo = (int)ceil(y[p] - r[p]), jhi = (int)floor(y[p] + r[p]);
             if (ilo < 0) ilo = 0;
@@ -199,7 +205,7 @@ int oracle_cube(int64_t n, const double* x, const double* y, const double* h_eff
                     double w = oracle_weight(kind[p], x[p] - i, y[p] - j, h_eff[p], gt[p], gnorm[p]);
                     if (w == 0.0) continue;
                     double* dst = mine + ((size_t)i * ny + j) * nc;
-                    for (int c = 0; c < nc; ++c) dst[c] += w * S[c];
+                    for (int c = c0; c <= c1; ++c) dst[c] += w * S[c];
                 }
             }
         }

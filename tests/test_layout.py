@@ -5,7 +5,37 @@ import pytest
 import torch
 
 from martini_amd import _lib
-from martini_amd.insertion import shuffle_records
+from martini_amd.insertion import layout_records, record_permutation, shuffle_records
+
+
+def _fmix32(x):
+    """murmur3's 32-bit finaliser in numpy uint64 arithmetic."""
+    m = np.uint64(0xFFFFFFFF)
+    x = x.astype(np.uint64) & m
+    x ^= x >> np.uint64(16)
+    x = (x * np.uint64(0x85EBCA6B)) & m
+    x ^= x >> np.uint64(13)
+    x = (x * np.uint64(0xC2B2AE35)) & m
+    return x ^ (x >> np.uint64(16))
+
+
+@pytest.mark.parametrize("n", [2, 1000, 65537])
+def test_record_permutation_is_the_hash_order(n):
+    """The permutation is the index order of a bijective 32-bit hash of the
+    index, restated here in numpy: the same order on any device."""
+    k = np.arange(n, dtype=np.uint64)
+    keys = _fmix32(_fmix32(k) ^ np.uint64(0x6D617274))
+    assert np.unique(keys).size == n
+    want = np.argsort(keys, kind="stable")
+    assert np.array_equal(record_permutation(n, "cpu").numpy(), want)
+
+
+def test_mfma_layout_keeps_the_given_order():
+    rec = np.zeros(100, dtype=_lib.PARTICLE_DTYPE)
+    rec["ix"] = np.arange(100)
+    t = torch.from_numpy(rec.view(np.uint8).copy())
+    assert layout_records(t, "mfma") is t
+    assert not torch.equal(layout_records(t, "plain"), t)
 
 
 def _shuffled_order(n):
