@@ -13,7 +13,10 @@
  *   - Return value: 0 on success, a negative MI_ERR_* code on failure.
  *     mi_strerror() turns it into text.  No exception crosses the ABI.
  *   - Nothing is allocated inside the calls: the caller passes the
- *     workspace, sized by mi_insert_workspace_size().
+ *     workspace, sized by mi_insert_workspace_size().  The kernel weight
+ *     tables are constant data in a static device array; the first call on a
+ *     device that reads them (mi_insert, mi_gather, mi_kernel_weights) fills
+ *     them on its stream and synchronises that stream once.
  *   - Calls are reentrant for distinct (workspace, stream) pairs.
  *   - The cube is FP32, row-major [nx][ny][nc] over the padded extent, and
  *     is ACCUMULATED (+=), as the reference accumulates into DataCube._array

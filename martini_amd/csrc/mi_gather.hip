@@ -221,11 +221,9 @@ __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item
 // tabulated kernels), so each device holds one copy, filled on first use.
 __device__ float4 g_wtab[WTAB_KERNELS * WTAB_N];
 
-__global__ void k_fill_wtab() { fill_wtab(g_wtab); }
+__global__ void k_fill_wtab(float4* __restrict__ tab) { fill_wtab(tab); }
 
-constexpr int MAX_DEVICES = 64;
-
-hipError_t weight_table(const float4** out) {
+hipError_t weight_table(hipStream_t s, const float4** out) {
     static std::mutex mu;
     static const float4* table[MAX_DEVICES] = {};
     int dev = 0;
@@ -234,18 +232,17 @@ hipError_t weight_table(const float4** out) {
     if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
     std::lock_guard<std::mutex> lock(mu);
     if (table[dev] == nullptr) {
+        // First use on this device: fill on the caller's stream and wait for it,
+        // so that work on any stream may read the table afterwards.
         void* p = nullptr;
         e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_wtab));
         if (e != hipSuccess) return e;
-        hipStream_t st;
-        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_fill_wtab, dim3((WTAB_KERNELS * WTAB_N + 255) / 256), dim3(256), 0, st);
+        hipLaunchKernelGGL(k_fill_wtab, dim3((WTAB_KERNELS * WTAB_N + 255) / 256), dim3(256), 0, s,
+                           static_cast<float4*>(p));
         e = hipGetLastError();
-        const hipError_t es = hipStreamSynchronize(st);
-        hipStreamDestroy(st);
         if (e != hipSuccess) return e;
-        if (es != hipSuccess) return es;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
         table[dev] = static_cast<const float4*>(p);
     }
     *out = table[dev];
@@ -273,7 +270,7 @@ hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_des
                         const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
     const float4* wtab = nullptr;
-    hipError_t e = weight_table(&wtab);
+    hipError_t e = weight_table(s, &wtab);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gather_plain, dim3((unsigned)L.items_max), dim3(NT), 0, s,
                        d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
@@ -295,7 +292,7 @@ hipError_t launch_weights(int kind, float gt, float ginv, const float* dx, const
                           const float* h, float* out, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const float4* wtab = nullptr;
-    hipError_t e = weight_table(&wtab);
+    hipError_t e = weight_table(s, &wtab);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_weights, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, kind, gt,
                        ginv, dx, dy, h, wtab, out, n);

@@ -76,9 +76,12 @@ hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc
 hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cube,
                            hipStream_t s);
 
-// The per-device weight tables (WTAB_KERNELS x WTAB_N, filled by fill_wtab on
-// the device's first use, synchronously on a stream of their own).
-hipError_t weight_table(const float4** out);
+// Devices the library keeps per-device state for (tables, scratch pointers).
+constexpr int MAX_DEVICES = 64;
+
+// The per-device weight tables (WTAB_KERNELS x WTAB_N), filled by fill_wtab on
+// the device's first use: launched on stream s, which is then synchronised.
+hipError_t weight_table(hipStream_t s, const float4** out);
 hipError_t launch_weights(int kind, float gt, float ginv, const float* dx, const float* dy,
                           const float* h, float* out, int64_t n, hipStream_t s);
 hipError_t launch_spectra(const mi_particle* d_p, int64_t n, const mi_cube_desc& d, float* out,

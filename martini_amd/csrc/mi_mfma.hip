@@ -187,7 +187,7 @@ hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc
                        const Layout& L, float* d_cube, hipStream_t s) {
     if (L.items_max <= 0) return hipSuccess;
     const float4* wtab = nullptr;
-    hipError_t e = weight_table(&wtab);
+    hipError_t e = weight_table(s, &wtab);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gather_mfma, dim3((unsigned)L.items_max), dim3(NT), 0, s,
                        d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
