@@ -88,15 +88,56 @@ def test_exact_kernels_against_quadrature(name, R):
     assert w == pytest.approx(ex, rel=1e-9)
 
 
-@pytest.mark.parametrize("R", [0.05, 0.2, 0.4, 0.5])
-def test_wc6_near_quadrature(R):
-    """WendlandC6 keeps the reference's rounded coefficients: close to, not
-    equal to, the exact line-of-sight integral in the core."""
+# The three coefficients of the reference's WendlandC6 antiderivative
+# (sph_kernels.py:576-678) that it gives rounded to four decimals, keyed by
+# (power of R, power of z, factor: 1 = sqrt(R^2 + z^2)), with the exact
+# rationals of the antiderivative of the 3D kernel.
+WC6_ROUNDED = {
+    (10, 1, 1): (24.7813, 793 / 32),
+    (8, 3, 1): (47.4792, 2279 / 48),
+    (6, 5, 1): (58.0167, 3481 / 60),
+}
+
+
+def _wc6_rounding_effect(R):
+    """What the three rounded coefficients add to the z-integral from 0 to
+    sqrt(1 - R^2), in closed form: sum of (rounded - exact) times the term's
+    difference between the two limits (each term vanishes at z = 0)."""
+    z = np.sqrt(1.0 - R * R)
+    S = np.sqrt(R * R + z * z)
+    return sum((c - ex) * R**pr * z**pz * S for (pr, pz, _), (c, ex) in WC6_ROUNDED.items())
+
+
+@pytest.mark.parametrize("R", [0.02, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 0.95, 0.99])
+def test_wc6_is_exact_integral_plus_its_rounding(R):
+    """The oracle's WendlandC6 pixel weight equals the exact line-of-sight
+    integral of the 3D kernel (quadrature) plus the closed-form effect of the
+    reference's three rounded coefficients, within 1e-11 of the central value.
+    The rounding is up to 1e-5 of it, so every other coefficient of the
+    39-term antiderivative, and every digit of the three rounded ones, is
+    pinned: a one-digit change anywhere moves the result by more than 1e-9."""
     zmax = np.sqrt(1 - R * R)
-    ex = 2 * quad(lambda z: float(O.W_wc6(np.array([np.sqrt(R * R + z * z)]))[0]), 0, zmax,
-                  epsabs=1e-15, limit=200)[0]
+    exact = quad(lambda z: float(O.W_wc6(np.array([np.sqrt(R * R + z * z)]))[0]), 0, zmax,
+                 epsabs=1e-16, epsrel=1e-14, limit=400)[0]
+    norm = 1365 / 64 / np.pi
+    want = 2 * (exact + norm * _wc6_rounding_effect(R))
     w = O.KernelSpec("wc6").integral(np.array([R]), np.array([0.0]), np.array([1.0]))[0]
-    assert w == pytest.approx(ex, rel=1e-3)
+    central = 2 * norm * 4 / 15
+    assert abs(w - want) <= 1e-11 * central
+    # without the rounding correction the difference is visible (R beyond the core)
+    if R >= 0.3:
+        assert abs(w - 2 * exact) > 1e-8 * central
+
+
+def test_wc6_rounding_pin_catches_a_digit():
+    """A last-digit change of one rounded coefficient (24.7813 -> 24.7814)
+    breaks the 1e-11 agreement of the test above."""
+    R = 0.9
+    zmax = np.sqrt(1 - R * R)
+    S = np.sqrt(R * R + zmax * zmax)
+    central = 2 * (1365 / 64 / np.pi) * 4 / 15
+    shift = 2 * (1365 / 64 / np.pi) * 1e-4 * R**10 * zmax * S
+    assert shift > 1e-9 * central
 
 
 @pytest.mark.parametrize("name,target,tol", [("wc2", 1.0, 1e-11), ("quartic", 1.0, 1e-11),

@@ -243,3 +243,74 @@ def test_c_cube_thread_count_invariant():
     a = cube_c.cube(P, nthreads=1)
     b = cube_c.cube(P, nthreads=4)
     np.testing.assert_allclose(b, a, rtol=1e-13, atol=1e-15 * a.max())
+
+
+def _windmill():
+    """The reference's four-armed windmill (tests/test_sources.py:36-45): 200
+    particles per arm in the x-y plane, rotating right-handed about z, with
+    speeds growing outwards and unit masses; its angular momentum points
+    along +z."""
+    arm_xyz = np.array([[1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0]], float)
+    arm_v = np.array([[0, 1, 0], [-1, 0, 0], [0, -1, 0], [1, 0, 0]], float)
+    k = np.arange(1, 201, dtype=float).reshape(200, 1, 1)
+    return np.vstack(arm_xyz * k), np.vstack(arm_v * k), np.ones(800)
+
+
+@pytest.mark.parametrize("axis,target", [("z", [0, 0, 1]), ("x", [1, 0, 0]), ("y", [0, 1, 0])])
+def test_L_align_windmill(axis, target):
+    """tests/test_sources.py:36-75 of the reference: the rotation takes the
+    windmill's angular momentum (+z) to the requested axis.  The reference
+    checks both input layouts: (3, N) coordinates give a matrix acting on
+    column vectors (rot @ z), (N, 3) coordinates its transpose acting on row
+    vectors (z @ rot.T); the oracle returns the (3, N) form.  A wrong axis
+    roll sends z to another axis and fails."""
+    xyz, vxyz, m = _windmill()
+    rot = O.L_align(xyz, vxyz, m, Laxis=axis)
+    np.testing.assert_allclose(rot @ np.array([0.0, 0.0, 1.0]), target, atol=1e-12)
+    np.testing.assert_allclose(np.array([[0.0, 0.0, 1.0]]) @ rot.T, [target], atol=1e-12)
+    np.testing.assert_allclose(rot @ rot.T, np.eye(3), atol=1e-12)
+    assert np.linalg.det(rot) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_L_align_product_matches_windmill():
+    """The product's L_align (x axis, as SPHSource.rotate uses it) sends the
+    windmill's angular momentum to +x, as the oracle's does."""
+    from martini_amd.sources import _L_align
+
+    xyz, vxyz, m = _windmill()
+    rot = _L_align(xyz, vxyz, m)
+    np.testing.assert_allclose(rot @ np.array([0.0, 0.0, 1.0]), [1.0, 0.0, 0.0], atol=1e-12)
+    np.testing.assert_allclose(rot, O.L_align(xyz, vxyz, m, Laxis="x"), atol=1e-13)
+
+
+@pytest.mark.parametrize("incl", [0.0, 30.0, -30.0])
+@pytest.mark.parametrize("az_rot", [0.0, 30.0, -30.0])
+@pytest.mark.parametrize("pa", [270.0, 300.0, 240.0])
+def test_rotate_L_coords(incl, az_rot, pa):
+    """tests/test_sources.py:465-501 of the reference: the L_coords rotation is
+    L_align (to x), then az_rot about x, incl about y, and the position angle
+    about x, offset by 90 deg for incl >= 0 and 270 deg otherwise (composed here
+    with scipy's rotations).  Independently of that composition, the windmill's
+    angular momentum must end at (cos|i|, -sin|i| cos pa, -sin|i| sin pa): at
+    |incl| from the line of sight (x), projected on the sky along the position
+    angle the same way for either sign of incl."""
+    from scipy.spatial.transform import Rotation
+
+    from martini_amd import L_coords, SPHSource
+
+    xyz, vxyz, m = _windmill()
+    rot = O.L_coords_matrix(xyz, vxyz, m, incl, az_rot, pa)
+    want = Rotation.from_euler("x", np.deg2rad(az_rot)).as_matrix() @ O.L_align(
+        xyz, vxyz, m, Laxis="x")
+    want = Rotation.from_euler("y", np.deg2rad(incl)).as_matrix() @ want
+    off = pa - 90.0 if incl >= 0 else pa - 270.0
+    want = Rotation.from_euler("x", np.deg2rad(off)).as_matrix() @ want
+    np.testing.assert_allclose(rot, want, atol=1e-13)
+    i, p = np.deg2rad(abs(incl)), np.deg2rad(pa)
+    np.testing.assert_allclose(rot @ np.array([0.0, 0.0, 1.0]),
+                               [np.cos(i), -np.sin(i) * np.cos(p), -np.sin(i) * np.sin(p)],
+                               atol=1e-12)
+    src = SPHSource(distance=3.0, L_coords=L_coords(incl=incl, az_rot=az_rot, pa=pa),
+                    T_g=np.full(800, 1e4), mHI_g=m, xyz_g=xyz, vxyz_g=vxyz,
+                    hsm_g=np.full(800, 1.0))
+    np.testing.assert_allclose(src.current_rotation, rot, atol=1e-13)
