@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define MI_ABI_VERSION 2
+#define MI_ABI_VERSION 3
 
 /* Error codes. */
 #define MI_OK 0
@@ -53,6 +53,12 @@ extern "C" {
 #define MI_KERNEL_QUARTIC 3
 #define MI_KERNEL_GAUSS 4
 #define MI_KERNEL_DIRAC 5
+/* Any other kernel (a user _BaseSPHKernel subclass, or a subclass with its own
+ * _kernel_integral, sph_kernels.py:51-337): the caller evaluates its
+ * _kernel_integral over each particle's masked pixels, as _px_weight does per
+ * pixel (sph_kernels.py:85-119, called at martini.py:275-277), into a device
+ * table of weights (mi_cube_desc.weights) that the insertion reads. */
+#define MI_KERNEL_TABLE 6
 
 /* Spectral models (martini/spectral_models.py): GaussianSpectrum :304,
  * DiracDeltaSpectrum :488. */
@@ -84,6 +90,12 @@ extern "C" {
  * model's _BaseSpectrum.spectral_function, martini/spectral_models.py:
  * 216-273, evaluated by the caller) channel c receives
  * amp * spectra[k0 * spectra_ld + c] for c in [c_lo, c_hi]; fu and s are unused.
+ *
+ * MI_KERNEL_TABLE particles: h holds, as the bits of a uint32, the offset in
+ * mi_cube_desc.weights of the particle's pixel weights, row-major over its
+ * footprint clipped to the cube, rows i_lo..i_hi by columns j_lo..j_hi (the
+ * pixels the mask admits); the weight of pixel (i, j) is
+ * weights[h + (i - i_lo) * (j_hi - j_lo + 1) + (j - j_lo)].
  */
 typedef struct mi_particle {
     int32_t ix, iy;
@@ -105,6 +117,9 @@ typedef struct mi_cube_desc {
     const float* spectra;    /* MI_SPECTRUM_TABLE: device table of spectra, one row per
                                 record's k0 (null otherwise) */
     int64_t spectra_ld;      /* its row stride in floats, >= nc */
+    const float* weights;    /* MI_KERNEL_TABLE particles: device table of pixel weights
+                                (null when no particle uses it) */
+    int64_t weights_n;       /* its length in floats */
 } mi_cube_desc;
 
 /* Library ABI version (MI_ABI_VERSION). */
@@ -163,7 +178,9 @@ int mi_gather(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
  * Synchronises the stream and reports the last binning in this workspace:
  * out[0] = pairs produced, out[1] = overflow flag, out[2] = work items,
  * out[3] = split work items (partial tiles).  Returns MI_ERR_WORKSPACE on
- * overflow.
+ * overflow, and MI_ERR_ARGUMENT when a record with a footprint has an
+ * unknown kernel kind or MI_KERNEL_TABLE weights beyond mi_cube_desc.weights
+ * (the binning then makes no work items: the cube is left untouched).
  */
 int mi_status(const mi_cube_desc* desc, int64_t n, int64_t n_pairs, const void* d_workspace,
               uint32_t out[4], void* stream);

@@ -18,8 +18,9 @@ MI_OK = 0
 MI_ERR_WORKSPACE = -2
 
 KERNEL_CUBIC, KERNEL_WC2, KERNEL_WC6, KERNEL_QUARTIC, KERNEL_GAUSS, KERNEL_DIRAC = range(6)
+KERNEL_TABLE = 6  # a kernel without a device form: weights evaluated by the host
 SPECTRUM_GAUSS, SPECTRUM_DIRAC, SPECTRUM_TABLE = 0, 1, 2
-ABI_VERSION = 2
+ABI_VERSION = 3
 METHOD_PLAIN_F32, METHOD_MFMA_BF16X3 = 0, 1
 METHODS = {"plain": METHOD_PLAIN_F32, "fp32": METHOD_PLAIN_F32,
            "mfma": METHOD_MFMA_BF16X3, "bf16x3": METHOD_MFMA_BF16X3}
@@ -39,7 +40,8 @@ class CubeDesc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nc", ctypes.c_int32),
                 ("spectrum", ctypes.c_int32), ("gauss_t", ctypes.c_float * 4),
                 ("gauss_inv_norm", ctypes.c_float * 4), ("spectra", ctypes.c_void_p),
-                ("spectra_ld", ctypes.c_int64)]
+                ("spectra_ld", ctypes.c_int64), ("weights", ctypes.c_void_p),
+                ("weights_n", ctypes.c_int64)]
 
 
 class Observation(ctypes.Structure):

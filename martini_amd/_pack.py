@@ -79,10 +79,96 @@ def spectral_fields(vlos_kms, sigma_kms, mHI, dist_mpc, datacube, spectrum_kind,
     return (k0.astype(np.int64), fu, s, amp, c_lo.astype(np.int64), c_hi.astype(np.int64))
 
 
+# Pairs of (particle, pixel) per call of a host-evaluated kernel integral.
+TABLE_CHUNK_PAIRS = 1 << 22
+
+
+def device_kind(K) -> int:
+    """The device form of a concrete kernel: its own kind when it is one of the
+    built-in kernels and keeps their integral, else MI_KERNEL_TABLE (a user
+    subclass of _BaseSPHKernel, or a subclass that overrides _kernel_integral:
+    its integral is evaluated on the host, as the reference evaluates it)."""
+    from .sph_kernels import _BaseSPHKernel
+
+    if K._mi_kind >= 0 and type(K)._kernel_integral is _BaseSPHKernel._kernel_integral:
+        return int(K._mi_kind)
+    return _lib.KERNEL_TABLE
+
+
+def table_weights(sph_kernel, kernels, which, kinds, x, y, ix, iy, fx, fy, rr, h_eff, nx, ny):
+    """Pixel weights of the particles whose kernel has no device form
+    (MI_KERNEL_TABLE), from the kernel's own ``_kernel_integral``.
+
+    Each such particle's weights cover its footprint clipped to the cube,
+    rows i_lo..i_hi by columns j_lo..j_hi -- exactly the pixels the
+    reference's square mask admits (martini/martini.py:271-274) -- row-major;
+    offsets[p] is where they start.  The integral is called as the reference
+    calls it from the pixel loop (martini.py:275-277): offsets dij = particle -
+    pixel centre in pixels, h = sm_lengths * rescale, and for a kernel that is
+    not adaptive the particle mask (here the particle index of each pair); an
+    adaptive kernel's tiers get no mask (sph_kernels.py:1358-1362).  Returns
+    (offsets, FP32 weights)."""
+    n = x.size
+    offsets = np.zeros(n, np.int64)
+    sel = np.flatnonzero((kinds == _lib.KERNEL_TABLE) & (rr >= 0))
+    if sel.size == 0:
+        return offsets, np.zeros(0, np.float32)
+    r = rr[sel].astype(np.int64)
+    fx32 = fx[sel].astype(np.float32)
+    fy32 = fy[sel].astype(np.float32)
+    # the device's footprint (mi_device.h footprint): ceil(x - r) from the FP32 fraction
+    i_lo = np.maximum(ix[sel] - r + (fx32 > 0), 0)
+    i_hi = np.minimum(ix[sel] + r, nx - 1)
+    j_lo = np.maximum(iy[sel] - r + (fy32 > 0), 0)
+    j_hi = np.minimum(iy[sel] + r, ny - 1)
+    ni = np.maximum(i_hi - i_lo + 1, 0)
+    nj = np.maximum(j_hi - j_lo + 1, 0)
+    cnt = ni * nj
+    starts = np.concatenate(([0], np.cumsum(cnt)))
+    total = int(starts[-1])
+    if total >= 1 << 32:
+        raise ValueError("host-evaluated kernel weights beyond 2^32 entries: insert the source "
+                         "in parts")
+    offsets[sel] = starts[:-1]
+    out = np.zeros(total, np.float32)
+    adaptive = len(kernels) > 1 or hasattr(sph_kernel, "kernel_indices")
+    a = 0
+    while a < sel.size:
+        # a run of particles with at most TABLE_CHUNK_PAIRS pairs (at least one particle)
+        b = int(np.searchsorted(starts, starts[a] + TABLE_CHUNK_PAIRS, side="right")) - 1
+        b = max(b, a + 1)
+        b = min(b, sel.size)
+        c = cnt[a:b]
+        m = int(c.sum())
+        if m:
+            loc = np.repeat(np.arange(a, b), c)
+            l = np.arange(m) - np.repeat(starts[a:b] - starts[a], c)
+            row = l // nj[loc]
+            i = i_lo[loc] + row
+            j = j_lo[loc] + (l - row * nj[loc])
+            part = sel[loc]
+            dij = np.vstack((x[part] - i, y[part] - j))
+            h = h_eff[part]
+            w = np.empty(m)
+            for ik in np.unique(which[part]):
+                km = which[part] == ik
+                K = kernels[ik]
+                if adaptive:
+                    w[km] = np.asarray(K._kernel_integral(dij[:, km], h[km]), float)
+                else:
+                    w[km] = np.asarray(K._kernel_integral(dij[:, km], h[km], mask=part[km]),
+                                       float)
+            out[starts[a]:starts[a] + m] = w
+        a = b
+    return offsets, out
+
+
 def kernel_fields(sph_kernel, pixcoords, nx, ny):
-    """ix, iy, fx, fy, h, r_kind per particle and the Gaussian (t, norm) table."""
+    """ix, iy, fx, fy, h, r_kind per particle, the Gaussian (t, norm) table and
+    the host-evaluated weights of kernels without a device form (None when
+    there are none); for those particles h holds their table offset's bits."""
     kernels, which = sph_kernel._device_kernels()
-    kinds = np.array([K._mi_kind for K in kernels], np.int64)[which]
+    kinds = np.array([device_kind(K) for K in kernels], np.int64)[which]
     gauss = []
     gidx_of_kernel = np.zeros(len(kernels), np.int64)
     for ik, K in enumerate(kernels):
@@ -125,7 +211,22 @@ def kernel_fields(sph_kernel, pixcoords, nx, ny):
         fy = np.where(dirac, 0.0, fy)
         rr = np.where(dirac, np.where(ok, 0.0, -1.0), rr)
     r_kind = (rr.astype(np.int64) << 8) | (gidx << 4) | kinds
-    return ix.astype(np.int64), iy.astype(np.int64), fx, fy, h_eff, r_kind, gauss
+    ix = ix.astype(np.int64)
+    iy = iy.astype(np.int64)
+    table = None
+    if np.any(kinds == _lib.KERNEL_TABLE):
+        offsets, weights = table_weights(sph_kernel, kernels, which, kinds, xs, ys, ix, iy, fx,
+                                         fy, rr, h_eff, nx, ny)
+        table = (kinds == _lib.KERNEL_TABLE, offsets, weights)
+    return ix, iy, fx, fy, h_eff, r_kind, gauss, table
+
+
+def set_table_offsets(rec: np.ndarray, table) -> None:
+    """Write the weight-table offsets of MI_KERNEL_TABLE records into their h
+    field, as uint32 bits (mi_particle.h)."""
+    use, offsets, _ = table
+    words = rec.view(np.uint32).reshape(rec.size, _lib.PARTICLE_DTYPE.itemsize // 4)
+    words[use, _lib.PARTICLE_DTYPE.fields["h"][1] // 4] = offsets[use].astype(np.uint32)
 
 
 def make_records(n, ix, iy, fx, fy, h, r_kind, k0, fu, s, amp, c_lo, c_hi) -> np.ndarray:
@@ -162,9 +263,11 @@ def table_fields(spectral_model, source, datacube, out_scale=1.0):
 def pack_particles(source, sph_kernel, spectral_model, datacube, out_scale=1.0):
     """Records and cube descriptor for an initialised (and pruned) source.
     For a table spectrum the descriptor carries the host table as
-    ``host_spectra``; the inserter uploads it and points the descriptor at it."""
+    ``host_spectra``, for kernels without a device form the host-evaluated
+    pixel weights as ``host_weights``; the inserter uploads them and points
+    the descriptor at them."""
     nx, ny, nc = datacube._array.shape[:3]
-    ix, iy, fx, fy, h, r_kind, gauss = kernel_fields(sph_kernel, source.pixcoords, nx, ny)
+    ix, iy, fx, fy, h, r_kind, gauss, ktable = kernel_fields(sph_kernel, source.pixcoords, nx, ny)
     sc = source.skycoords
     table = None
     if spectral_model._mi_kind == _lib.SPECTRUM_TABLE:
@@ -179,6 +282,9 @@ def pack_particles(source, sph_kernel, spectral_model, datacube, out_scale=1.0):
                           spectra_ld=nc if table is not None else 0)
     if table is not None:
         desc.host_spectra = table
+    if ktable is not None:
+        set_table_offsets(rec, ktable)
+        desc.host_weights = ktable[2]
     return rec, desc
 
 

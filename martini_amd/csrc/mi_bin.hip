@@ -89,17 +89,38 @@ __device__ inline uint32_t spread4(uint32_t v) {
     return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2) | ((v & 8u) << 3);
 }
 
+// Kernel families in a cell's sort order: tabulated closed forms first (the
+// item stages the first one's table in LDS), then Gaussian, Dirac, and the
+// caller's weight tables.
 __device__ inline uint32_t kernel_group(int kind) {
-    return kind == MI_KERNEL_GAUSS ? 1u : (kind == MI_KERNEL_DIRAC ? 2u : 0u);
+    return kind == MI_KERNEL_GAUSS ? 1u
+           : kind == MI_KERNEL_DIRAC ? 2u
+           : kind == MI_KERNEL_TABLE ? 3u : 0u;
+}
+
+// Whether a record with a footprint can be inserted: a known kernel kind, and
+// for MI_KERNEL_TABLE weights that lie inside the descriptor's table.
+__device__ inline bool record_ok(const mi_particle& p, const Grid& g, const float* uw,
+                                 int64_t uw_n) {
+    const int kind = kind_of(p.r_kind);
+    if (kind < MI_KERNEL_TABLE) return true;
+    if (kind > MI_KERNEL_TABLE || uw == nullptr) return false;
+    int i_lo, i_hi, j_lo, j_hi;
+    if (!footprint(p, g.nx, g.ny, i_lo, i_hi, j_lo, j_hi)) return true;
+    const int64_t end = (int64_t)table_offset(p) + (int64_t)(i_hi - i_lo + 1) * (j_hi - j_lo + 1);
+    return end <= uw_n;
 }
 
 __global__ void k_count(const mi_particle* __restrict__ P, int64_t n, Grid g,
-                        uint32_t* __restrict__ counts) {
+                        const float* __restrict__ uw, int64_t uw_n,
+                        uint32_t* __restrict__ counts, uint32_t* __restrict__ status) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     mi_particle p = P[k];
     int a, b, c, d, e, f;
-    counts[k] = cells_of(p, g, a, b, c, d, e, f);
+    const uint32_t cells = cells_of(p, g, a, b, c, d, e, f);
+    counts[k] = cells;
+    if (cells > 0 && !record_ok(p, g, uw, uw_n)) status[ST_BADREC] = 1u;  // same value from all
 }
 
 // Writes the (key, particle) pairs of each particle at its scanned offset.
@@ -192,7 +213,7 @@ __global__ __launch_bounds__(SCAN_T) void k_item_scan(const uint32_t* __restrict
     const int t = threadIdx.x;
     const int64_t per = (ncells + SCAN_T - 1) / SCAN_T;
     const int64_t c0 = min((int64_t)t * per, ncells), c1 = min(c0 + per, ncells);
-    const bool overflow = status[ST_OVERFLOW] != 0u;
+    const bool overflow = status[ST_OVERFLOW] != 0u || status[ST_BADREC] != 0u;
     const uint32_t L = (uint32_t)item_len;
     auto items_of = [&](int64_t c) {
         const uint32_t s0 = start[c], e0 = end[c];
@@ -255,7 +276,8 @@ __global__ __launch_bounds__(64) void k_build_items(const uint32_t* __restrict__
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
-                         char* ws, const Layout& L, bool full_order, hipStream_t s) {
+                         const mi_cube_desc& d, char* ws, const Layout& L, bool full_order,
+                         hipStream_t s) {
     uint32_t* counts = (uint32_t*)(ws + L.counts);
     uint32_t* offsets = (uint32_t*)(ws + L.offsets);
     uint32_t* keys = (uint32_t*)(ws + L.keys);
@@ -275,7 +297,8 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
     hipLaunchKernelGGL(k_bin_init, dim3(blocks_for(g.ncells > 8 ? g.ncells : 8, T)), dim3(T), 0, s,
                        status, cstart, cend, g.ncells);
     if (n > 0) {
-        hipLaunchKernelGGL(k_count, dim3(blocks_for(n, T)), dim3(T), 0, s, d_p, n, g, counts);
+        hipLaunchKernelGGL(k_count, dim3(blocks_for(n, T)), dim3(T), 0, s, d_p, n, g, d.weights,
+                           d.weights_n, counts, status);
         hipError_t e = rocprim::exclusive_scan(temp, tb, counts, offsets, 0u, (size_t)n,
                                                rocprim::plus<uint32_t>(), s);
         if (e != hipSuccess) return e;

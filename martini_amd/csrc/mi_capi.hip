@@ -16,6 +16,7 @@ static bool desc_ok(const mi_cube_desc* d) {
         d->spectrum != MI_SPECTRUM_TABLE)
         return false;
     if (d->spectrum == MI_SPECTRUM_TABLE && (!d->spectra || d->spectra_ld < d->nc)) return false;
+    if (d->weights_n < 0 || (d->weights_n > 0 && !d->weights)) return false;
     return true;
 }
 
@@ -86,7 +87,7 @@ int mi_bin(const mi_particle* d_particles, int64_t n, int64_t n_pairs, const mi_
     Layout L;
     int rc = prepare(d_particles, n, n_pairs, desc, d_workspace, workspace_bytes, &g, &L);
     if (rc) return rc;
-    if (bin_particles(d_particles, n, n_pairs, g, (char*)d_workspace, L,
+    if (bin_particles(d_particles, n, n_pairs, g, *desc, (char*)d_workspace, L,
                       method == MI_METHOD_MFMA_BF16X3, (hipStream_t)stream) != hipSuccess)
         return MI_ERR_DEVICE;
     return MI_OK;
@@ -129,7 +130,8 @@ int mi_status(const mi_cube_desc* desc, int64_t n, int64_t n_pairs, const void* 
     out[1] = st[ST_OVERFLOW];
     out[2] = st[ST_ITEMS];
     out[3] = st[ST_MULTI];
-    return st[ST_OVERFLOW] ? MI_ERR_WORKSPACE : MI_OK;
+    if (st[ST_OVERFLOW]) return MI_ERR_WORKSPACE;
+    return st[ST_BADREC] ? MI_ERR_ARGUMENT : MI_OK;
 }
 
 int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,

@@ -40,6 +40,12 @@ __host__ __device__ inline int item_length(int64_t n_pairs) {
 constexpr int KEY_LOCAL_BITS = 10;
 
 __host__ __device__ inline int kind_of(int32_t r_kind) { return r_kind & 15; }
+// MI_KERNEL_TABLE particles carry the offset of their weights in the h field.
+__host__ __device__ inline uint32_t table_offset(const mi_particle& p) {
+    uint32_t u;
+    __builtin_memcpy(&u, &p.h, 4);
+    return u;
+}
 __host__ __device__ inline int gidx_of(int32_t r_kind) { return (r_kind >> 4) & 3; }
 __host__ __device__ inline int range_of(int32_t r_kind) { return r_kind >> 8; }
 

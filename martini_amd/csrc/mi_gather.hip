@@ -113,7 +113,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             fill_gauss_tables(cs, gt);
             __syncthreads();
         }
-        fill_weights(cs, x0, y0, wtab, tkind, Tk, gt, [&](int k, int px, float w) { Wt[k][px] = w; });
+        fill_weights(cs, x0, y0, wtab, tkind, Tk, gt, d.weights, [&](int k, int px, float w) { Wt[k][px] = w; });
         fill_spectra(cs, nk, c0, g.nc, d, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
         if (kmask == 0u) continue;

@@ -56,7 +56,9 @@ struct Layout {
 };
 
 // status[0]: number of pairs the device produced; status[1]: overflow flag.
-enum { ST_PAIRS = 0, ST_OVERFLOW = 1, ST_ITEMS = 2, ST_MULTI = 3 };
+// status[4]: a record with a footprint has an unknown kernel kind or table
+// weights beyond the descriptor's table (no work items are then made).
+enum { ST_PAIRS = 0, ST_OVERFLOW = 1, ST_ITEMS = 2, ST_MULTI = 3, ST_BADREC = 4 };
 
 hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L);
 
@@ -66,7 +68,8 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L);
 // each chunk over its tile, and are sorted on the top 16 bits (two radix
 // passes) when the key is longer.
 hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
-                         char* ws, const Layout& L, bool full_order, hipStream_t s);
+                         const mi_cube_desc& d, char* ws, const Layout& L, bool full_order,
+                         hipStream_t s);
 
 // Gather over work items (plain FP32 or MFMA), then the partial-tile reduction.
 hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,

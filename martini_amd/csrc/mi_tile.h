@@ -24,12 +24,25 @@ constexpr int NT = 256;   // threads per work group
 
 // What the W walk needs of a particle, read with two 16-byte loads whenever
 // a thread moves on to the next particle.
+// For MI_KERNEL_TABLE particles px and py hold instead, as int64 bits
+// (table_walk), the index of pixel (0, 0) in the caller's weight table and the
+// row stride of the particle's weights.
 struct __attribute__((aligned(16))) WalkRec {
     double px, py;  // particle position ix + fx, iy + fy (exact in FP64)
     double fa;      // WFactors::a: 1/h^2, or 1/(h sig)^2 for the Gaussian
     int32_t ij;     // first row | first column << 16 of the clipped footprint
     int32_t kw;     // kernel kind | footprint width << 8
 };
+
+// Weight-table walk of an MI_KERNEL_TABLE particle: weights[base + i * stride + j]
+// is its weight at pixel (i, j).
+struct TableWalk {
+    int64_t base, stride;
+};
+
+__device__ inline TableWalk table_walk(const WalkRec& w) {
+    return TableWalk{__double_as_longlong(w.px), __double_as_longlong(w.py)};
+}
 
 struct ChunkState {
     mi_particle p[KC];
@@ -132,7 +145,11 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
     const mi_particle p = unpack_record(r);
     int cnt = 0, ilo = 1, ihi = 0, jlo = 1, jhi = 0;
     uint32_t rows = 0, chans = 0;
+    TableWalk tw{0, 0};
     if (valid && footprint(p, g.nx, g.ny, ilo, ihi, jlo, jhi)) {
+        // a table particle's weights are laid out over its cube-clipped footprint
+        tw.stride = jhi - jlo + 1;
+        tw.base = (int64_t)table_offset(p) - (int64_t)ilo * tw.stride - jlo;
         ilo = max(ilo, x0);
         ihi = min(ihi, x0 + TILE - 1);
         jlo = max(jlo, y0);
@@ -172,6 +189,10 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
             WalkRec w;
             w.px = (double)p.ix + (double)p.fx;
             w.py = (double)p.iy + (double)p.fy;
+            if (kind == MI_KERNEL_TABLE) {
+                w.px = __longlong_as_double(tw.base);
+                w.py = __longlong_as_double(tw.stride);
+            }
             w.fa = f.a;
             w.ij = ilo | (jlo << 16);
             w.kw = kind | (width << 8);
@@ -253,7 +274,7 @@ template <class Store>
 __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                                     const float4* __restrict__ wtab, int tab_kind,
                                     const float4* __restrict__ tab, const GaussTables& gt,
-                                    Store store) {
+                                    const float* __restrict__ uw, Store store) {
     const int t = threadIdx.x;
     const bool active = t < cs.toff[KC];
     int k = 0;
@@ -304,6 +325,19 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
         return;
     }
 
+    if (kind == MI_KERNEL_TABLE) {
+        // the caller's weights (binning checked that they lie inside the table)
+        const TableWalk tw = table_walk(w);
+        for (int l = l0; l < l1; ++l) {
+            const int i = x0 + ti + a, j = y0 + tj + b;
+            store(k, (ti + a) * TILE + tj + b, uw[tw.base + (int64_t)i * tw.stride + j]);
+            if (++b == width) {
+                b = 0;
+                ++a;
+            }
+        }
+        return;
+    }
     if (kind == MI_KERNEL_GAUSS || !active) {
         // Gaussian particles: the separable factors come from the chunk's tables
         const double fe = cs.ge[k];

@@ -64,14 +64,24 @@ class Inserter:
                 d_spectra = torch.from_numpy(np.ascontiguousarray(host, np.float32)).to(self.device)
                 desc.spectra = d_spectra.data_ptr()
                 desc.spectra_ld = host.shape[1]
+        d_weights = None
+        host_w = getattr(desc, "host_weights", None)
+        if host_w is not None:
+            d_weights = torch.from_numpy(np.ascontiguousarray(host_w, np.float32)).to(self.device)
+            desc.weights = d_weights.data_ptr() if host_w.size else None
+            desc.weights_n = int(host_w.size)
         n_pairs = _lib.count_pairs(records, desc)
         d_rec = self.upload(records, method)
         if d_cube is None:
             d_cube = torch.zeros(desc.nx * desc.ny * desc.nc, dtype=torch.float32, device=self.device)
         self.insert_device(d_rec, n, n_pairs, desc, d_cube, method=method)
-        if d_spectra is not None:
-            torch.cuda.current_stream(self.device).synchronize()  # the table is read until here
-            desc.spectra = None
+        if d_spectra is not None or d_weights is not None:
+            torch.cuda.current_stream(self.device).synchronize()  # the tables are read until here
+            if d_spectra is not None:
+                desc.spectra = None
+            if d_weights is not None:
+                desc.weights = None
+                desc.weights_n = 0
         return d_cube
 
 

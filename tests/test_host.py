@@ -133,7 +133,7 @@ def test_record_dirac_pixel():
     k = K.DiracDeltaKernel()
     k._init_sm_lengths(src, dc)
     k._init_sm_ranges()
-    ix, iy, fx, fy, h, r_kind, _ = _pack.kernel_fields(k, src.pixcoords, 8, 8)
+    ix, iy, fx, fy, h, r_kind, _, _ = _pack.kernel_fields(k, src.pixcoords, 8, 8)
     assert list(ix[[0, 2, 4]]) == [2, 3, 0]
     assert list(r_kind[[0, 2, 4]] >> 8) == [0, 0, 0]
     assert (r_kind[1] >> 8) == -1  # exactly on a pixel edge: no pixel (strict <)
@@ -198,7 +198,7 @@ def test_library_exports_every_declared_symbol():
     L = ctypes.CDLL(_lib.LIB_PATH)
     for name in sorted(names):
         assert hasattr(L, name), name
-    assert _lib.lib().mi_abi_version() == _lib.ABI_VERSION == 2
+    assert _lib.lib().mi_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_abi_argument_errors():
