@@ -41,6 +41,10 @@ extern "C" {
 #define MI_ERR_DEVICE (-3)     /* a HIP runtime call failed */
 #define MI_ERR_UNSUPPORTED (-4) /* kernel/spectrum kind not handled */
 #define MI_ERR_LIMIT (-5)      /* input exceeds a documented size limit */
+#define MI_ERR_SCRATCH (-6)    /* input needs more than MI_SCRATCH_BYTES (mi_insert_noalloc) */
+
+/* Bytes of the static per-device scratch that mi_insert_noalloc works in. */
+#define MI_SCRATCH_BYTES ((size_t)256 << 20)
 
 /* Kernel kinds (low 4 bits of mi_particle.r_kind).  Concrete kernels of
  * martini/sph_kernels.py: _CubicSplineKernel :724, _WendlandC2Kernel :340,
@@ -152,6 +156,24 @@ int mi_insert_workspace_size(const mi_cube_desc* desc, int64_t n, int64_t n_pair
 int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
               const mi_cube_desc* desc, float* d_cube, void* d_workspace,
               size_t workspace_bytes, int method, void* stream);
+
+/*
+ * mi_insert without a caller workspace: the same binning, gather and status
+ * check, working in a static device array of MI_SCRATCH_BYTES (256 MiB) that
+ * the library holds on each GPU it runs on until the process ends.  Nothing is
+ * allocated: no hipMalloc, no allocating library routine.  The caller passes
+ * the particles and the cube as for mi_insert.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Returns MI_ERR_SCRATCH, before any launch, when this input's workspace
+ * (mi_insert_workspace_size) exceeds MI_SCRATCH_BYTES.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (other threads,
+ * other streams) race and must be serialised by the caller.  The first call on
+ * a device resolves the scratch's device address and synchronises the stream
+ * (as mi_insert does after binning).
+ */
+int mi_insert_noalloc(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+                      const mi_cube_desc* desc, float* d_cube, int method, void* stream);
 
 /*
  * The two stages of mi_insert, asynchronous (no host synchronisation), for

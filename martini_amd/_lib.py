@@ -107,6 +107,8 @@ def lib() -> ctypes.CDLL:
     L.mi_insert.restype = ctypes.c_int
     L.mi_insert.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, vp, ctypes.c_size_t,
                             ctypes.c_int, vp]
+    L.mi_insert_noalloc.restype = ctypes.c_int
+    L.mi_insert_noalloc.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, ctypes.c_int, vp]
     L.mi_bin.restype = ctypes.c_int
     L.mi_bin.argtypes = [vp, i64, i64, ctypes.POINTER(CubeDesc), vp, ctypes.c_size_t,
                          ctypes.c_int, vp]
@@ -195,6 +197,20 @@ def insert(d_records, n: int, n_pairs: int, desc: CubeDesc, d_cube, d_ws, method
                          ctypes.c_void_p(d_cube.data_ptr()), ctypes.c_void_p(d_ws.data_ptr()),
                          d_ws.numel() * d_ws.element_size(), m, stream_ptr(stream))
     check(rc, "mi_insert")
+
+
+SCRATCH_BYTES = 256 << 20  # MI_SCRATCH_BYTES
+MI_ERR_SCRATCH = -6
+
+
+def insert_noalloc(d_records, n: int, n_pairs: int, desc: CubeDesc, d_cube, method="plain",
+                   stream=None) -> int:
+    """mi_insert_noalloc: cube += insertion in the library's static scratch (no
+    workspace, nothing allocated); returns the status code unchanged."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    return int(lib().mi_insert_noalloc(ctypes.c_void_p(d_records.data_ptr()), n, n_pairs,
+                                       ctypes.byref(desc), ctypes.c_void_p(d_cube.data_ptr()), m,
+                                       stream_ptr(stream)))
 
 
 def bin(d_records, n: int, n_pairs: int, desc: CubeDesc, d_ws, stream=None,

@@ -3,11 +3,40 @@
 // stage sequence bin -> gather -> reduce.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "../../include/martini_mi355x.h"
 #include "mi_device.h"
 #include "mi_internal.h"
 
 using namespace mi;
+
+// Workspace of mi_insert_noalloc: a static device array, one per GPU, held
+// until the process ends.  Host code never passes this name to a kernel or a
+// copy: it uses the device address that hipGetSymbolAddress returns for the
+// current device (scratch_address).
+static __device__ __align__(256) unsigned char scratch[MI_SCRATCH_BYTES];
+
+// The device address of `scratch` on the current device, resolved once per
+// device with that device current.
+static hipError_t scratch_address(void** out) {
+    static std::mutex mu;
+    static void* ptr[MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        ptr[dev] = p;
+    }
+    *out = ptr[dev];
+    return hipSuccess;
+}
 
 static bool desc_ok(const mi_cube_desc* d) {
     if (!d) return false;
@@ -40,6 +69,9 @@ const char* mi_strerror(int code) {
     case MI_ERR_DEVICE: return "HIP runtime error";
     case MI_ERR_UNSUPPORTED: return "unsupported kernel, spectrum or method";
     case MI_ERR_LIMIT: return "input exceeds a size limit of the library";
+    case MI_ERR_SCRATCH:
+        return "input needs more workspace than the static scratch of mi_insert_noalloc "
+               "(MI_SCRATCH_BYTES)";
     }
     return "unknown error";
 }
@@ -146,6 +178,19 @@ int mi_insert(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
     if (rc) return rc;
     uint32_t st[4];
     return mi_status(desc, n, n_pairs, d_workspace, st, stream);
+}
+
+int mi_insert_noalloc(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
+                      const mi_cube_desc* desc, float* d_cube, int method, void* stream) {
+    if (!d_cube) return MI_ERR_ARGUMENT;
+    if (method != MI_METHOD_PLAIN_F32 && method != MI_METHOD_MFMA_BF16X3) return MI_ERR_UNSUPPORTED;
+    size_t need = 0;
+    int rc = mi_insert_workspace_size(desc, n, n_pairs, &need);
+    if (rc) return rc;
+    if (need > MI_SCRATCH_BYTES) return MI_ERR_SCRATCH;
+    void* ws = nullptr;
+    if (scratch_address(&ws) != hipSuccess) return MI_ERR_DEVICE;
+    return mi_insert(d_particles, n, n_pairs, desc, d_cube, ws, MI_SCRATCH_BYTES, method, stream);
 }
 
 int mi_kernel_weights(int kind, float gauss_t, float gauss_inv_norm, const float* d_dx,
