@@ -66,6 +66,10 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     L->item_off = take(4 * nc1);
     L->multi_off = take(4 * nc1);
     L->items = take(sizeof(Item) * (size_t)L->items_max);
+    L->item_keys = take(4 * (size_t)L->items_max);
+    L->item_keys2 = take(4 * (size_t)L->items_max);
+    L->item_idx = take(4 * (size_t)L->items_max);
+    L->item_order = take(4 * (size_t)L->items_max);
     L->partials = take(sizeof(float) * (size_t)L->multi_max * TILE_PX * CBLOCK);
     L->status = take(4 * 8);
     // rocPRIM temporary storage: the larger of its two uses.
@@ -77,7 +81,12 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     e = rocprim::radix_sort_pairs(nullptr, t2, nul, nul, nul, nul, np, 0,
                                   g.cell_bits + KEY_LOCAL_BITS, 0);
     if (e != hipSuccess) return e;
-    const size_t tb = t1 > t2 ? t1 : t2;
+    size_t t3 = 0;
+    e = rocprim::radix_sort_pairs(nullptr, t3, nul, nul, nul, nul, (size_t)L->items_max, 0,
+                                  ITEM_KEY_BITS, 0);
+    if (e != hipSuccess) return e;
+    size_t tb = t1 > t2 ? t1 : t2;
+    tb = tb > t3 ? tb : t3;
     L->temp_bytes = tb;
     L->temp = take(tb);
     L->total = off;
@@ -273,6 +282,22 @@ __global__ __launch_bounds__(64) void k_build_items(const uint32_t* __restrict__
     }
 }
 
+// Launch order of the work items: longest first (ties in cell order), so that
+// the gathers' work groups, dispatched in order, take the long items early and
+// the short ones fill the tail.  The key is ITEM_MAX_LEN - length; slots past
+// the item count sort last.  Only the schedule changes: every cell still sums
+// its items in item order, so the cube is bitwise the same in any order.
+__global__ void k_item_keys(const Item* __restrict__ items, const uint32_t* __restrict__ status,
+                            int64_t items_max, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ idx) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= items_max) return;
+    const uint32_t n_items = status[ST_ITEMS];
+    keys[j] = (uint32_t)j < n_items ? (uint32_t)(ITEM_MAX - (items[j].end - items[j].begin))
+                                    : (1u << (ITEM_KEY_BITS - 1));
+    idx[j] = (uint32_t)j;
+}
+
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
@@ -327,6 +352,14 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
                        L.item_len, status, ioff, moff);
     hipLaunchKernelGGL(k_build_items, dim3((unsigned)g.ncells), dim3(64), 0, s, cstart, cend,
                        L.item_len, ioff, moff, items);
+    uint32_t* ikeys = (uint32_t*)(ws + L.item_keys);
+    uint32_t* iidx = (uint32_t*)(ws + L.item_idx);
+    hipLaunchKernelGGL(k_item_keys, dim3(blocks_for(L.items_max, T)), dim3(T), 0, s, items, status,
+                       L.items_max, ikeys, iidx);
+    hipError_t e = rocprim::radix_sort_pairs(temp, tb, ikeys, (uint32_t*)(ws + L.item_keys2), iidx,
+                                             (uint32_t*)(ws + L.item_order), (size_t)L.items_max,
+                                             0, ITEM_KEY_BITS, s);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

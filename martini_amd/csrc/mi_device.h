@@ -28,8 +28,12 @@ constexpr int CBLOCK = 64;
 // ITEM_MIN and ITEM_MAX particles each), so a given input always sums in the
 // same order.
 constexpr int ITEM_MIN = 256;
-constexpr int ITEM_MAX = 2048;
+constexpr int ITEM_MAX = 4096;
 constexpr int ITEM_TARGET = 2048;
+// Bits of the launch-order key of an item (ITEM_MAX - length, or the padding
+// key 2^(ITEM_KEY_BITS - 1) of slots past the item count).
+constexpr int ITEM_KEY_BITS = 15;
+static_assert(ITEM_MAX < (1 << (ITEM_KEY_BITS - 1)), "item lengths fit the launch-order key");
 
 __host__ __device__ inline int item_length(int64_t n_pairs) {
     int64_t len = (n_pairs / ITEM_TARGET + 31) / 32 * 32;

@@ -45,7 +45,9 @@ struct Layout {
     size_t cell_start, cell_end;       // uint32[ncells] each
     size_t item_off, multi_off;        // uint32[ncells + 1] each: exclusive scans of the
                                        // items and the partial-tile slots per cell
-    size_t items;                      // int4[items_max]
+    size_t items;                      // int4[items_max], in cell order
+    size_t item_keys, item_keys2;      // uint32[items_max]: sort keys of the launch order
+    size_t item_idx, item_order;       // uint32[items_max]: item index; launch order
     size_t partials;                   // float[multi_max * TILE_PX * CBLOCK]
     size_t status;                     // uint32[8]
     size_t temp;                       // rocPRIM temporary storage

@@ -62,13 +62,12 @@ __device__ inline bf16x8 get8(const char* tile, int row, int kg) {
 // groups share a CU (LDS 48 KB each).
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_gather_mfma(
     const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
-    const Item* __restrict__ items,
+    const Item* __restrict__ items, const uint32_t* __restrict__ item_order,
     const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
     float* __restrict__ cube, float* __restrict__ partials) {
     if (status[ST_OVERFLOW]) return;
-    const int item_id = blockIdx.x;
-    if ((uint32_t)item_id >= status[ST_ITEMS]) return;
-    const Item it = items[item_id];
+    if (blockIdx.x >= status[ST_ITEMS]) return;
+    const Item it = items[item_order[blockIdx.x]];  // longest items first
     const CellPos cp = cell_pos(it.cell, g);
     const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
 
@@ -191,6 +190,7 @@ hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gather_mfma, dim3((unsigned)L.items_max), dim3(NT), 0, s,
                        d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
+                       (const uint32_t*)(ws + L.item_order),
                        (const uint32_t*)(ws + L.status), wtab, g, d,
                        d_cube, (float*)(ws + L.partials));
     return hipGetLastError();

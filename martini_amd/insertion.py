@@ -143,14 +143,44 @@ def shuffle_records(d_rec):
     return words.index_select(0, idx).view(-1).view(torch.uint8)
 
 
+def _spread4(v):
+    """Bits 0..3 of v spread to the even bit positions 0, 2, 4, 6."""
+    v = v & 15
+    return (v & 1) | ((v & 2) << 1) | ((v & 4) << 2) | ((v & 8) << 3)
+
+
+def spatial_records(d_rec):
+    """Records (uint8 tensor of n * 48 bytes) sorted by 16 x 16-pixel tile of
+    their pixel (ix, iy), tiles in the binning's cell order (x-major), and by
+    the pixel's Z order inside the tile: the order in which the MFMA form's
+    binning lists each cell's particles.  A chunk of 32 consecutive particles of
+    a cell then reads neighbouring records, whole cache lines instead of one
+    48-byte record per line.  Equal keys keep the given order (a stable sort),
+    so the layout is the same on every device."""
+    import torch
+
+    n = d_rec.numel() // _lib.PARTICLE_DTYPE.itemsize
+    if n <= 1:
+        return d_rec
+    words = d_rec.view(torch.int32).view(n, _lib.PARTICLE_DTYPE.itemsize // 4)
+    ix = words[:, 0].to(torch.int64)
+    iy = words[:, 1].to(torch.int64)
+    off = 1 << 17  # |ix|, |iy| < 2^21: tile indices shifted to be non-negative
+    key = ((((ix >> 4) + off) << 18) | ((iy >> 4) + off)) << 8
+    key = key | _spread4(ix) | (_spread4(iy) << 1)
+    idx = torch.argsort(key, stable=True)
+    return words.index_select(0, idx).view(-1).view(torch.uint8)
+
+
 def layout_records(d_rec, method="plain"):
     """The records' HBM layout for a gather method: shuffled for the FP32 form
-    (``shuffle_records``), as given for the MFMA form, whose binning orders each
-    cell's particles fully (pixel Z order) so the record order does not reach
-    its chunks."""
+    (``shuffle_records``: its chunks should spread over the tile), in cell and
+    pixel Z order for the MFMA form (``spatial_records``: its chunks are
+    compact, and neighbouring reads then share cache lines).  The inserted cube
+    depends on the layout only through the FP32 summation order."""
     if _lib.METHODS[method] == _lib.METHOD_PLAIN_F32:
         return shuffle_records(d_rec)
-    return d_rec
+    return spatial_records(d_rec)
 
 
 _default = {}
