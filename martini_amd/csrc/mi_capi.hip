@@ -271,7 +271,7 @@ int mi_convolve_beam(const float* d_in, float* d_out, int32_t nx, int32_t ny, in
     if (!d_in || !d_out || !d_kernel || d_in == d_out) return MI_ERR_ARGUMENT;
     if (nx <= 0 || ny <= 0 || nc <= 0 || kx <= 0 || ky <= 0) return MI_ERR_ARGUMENT;
     if ((kx & 1) == 0 || (ky & 1) == 0) return MI_ERR_ARGUMENT;
-    if (nx > 32767 || ny > 32767 || beam_lds_bytes(kx, ky) > 64 * 1024) return MI_ERR_LIMIT;
+    if (nx > 32767 || ny > 32767 || kx > 32767 || ky > 32767) return MI_ERR_LIMIT;
     if (convolve_beam(d_in, d_out, nx, ny, nc, d_kernel, kx, ky, (hipStream_t)stream) !=
         hipSuccess)
         return MI_ERR_DEVICE;

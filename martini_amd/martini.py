@@ -98,6 +98,8 @@ class _BaseMartini:
         px2 = dc.px_size ** 2
         rec, desc = _pack.pack_particles(self.source, self.sph_kernel, self.spectral_model, dc,
                                          out_scale=1.0 / px2)
+        # the cube is empty unless something was inserted or added before
+        was_empty = not np.any(dc._array)
         d_cube = default_inserter().insert(rec, desc, method=method)
         added = d_cube.cpu().numpy().astype(np.float64).reshape(dc._array.shape[:3])
         if dc._array_unit == "Jy/pix2":
@@ -107,6 +109,11 @@ class _BaseMartini:
             dc._array[..., 0] += added
         else:
             dc._array += added
+        # The device cube equals the host cube while this array object holds
+        # exactly the insertion (FP32 values widened to FP64); convolve_beam
+        # then starts from it.  The methods of this class replace the array
+        # object whenever they change the cube.
+        self._resident = (d_cube, dc._array) if was_empty else None
         self._report(quiet)
 
     def _report(self, quiet) -> None:
@@ -131,12 +138,34 @@ class _BaseMartini:
 
     def reset(self) -> None:
         """A zero cube of the same geometry, padded for the beam (martini.py:409-425)."""
+        self._resident = None
         dc = self._datacube
         dc.padx = dc.pady = 0
         dc._array = np.zeros(dc._shape())
         dc._array_unit = "Jy/pix2"
         if self.beam is not None:
             dc.add_pad(self.beam.needs_pad())
+
+
+# A beam image is convolved as two one-dimensional passes when its second
+# singular value is below this fraction of the first: the outer product of the
+# two factors then equals the image to FP64 rounding.
+SEPARABLE_RTOL = 1e-12
+
+
+def beam_factors(kernel):
+    """(column factor u, row factor v) with kernel = outer(u, v) when the
+    beam image is separable to FP64 precision (a circular or axis-aligned
+    Gaussian beam's image is: the reference builds it from a tensor-product
+    spline of a separable function), else None."""
+    K = np.asarray(kernel, float)
+    U, S, Vt = np.linalg.svd(K)
+    if S[0] <= 0 or (S.size > 1 and S[1] > SEPARABLE_RTOL * S[0]):
+        return None
+    u, v = U[:, 0] * np.sqrt(S[0]), Vt[0] * np.sqrt(S[0])
+    if u.sum() < 0:
+        u, v = -u, -v
+    return u, v
 
 
 class Martini(_BaseMartini):
@@ -165,11 +194,24 @@ class Martini(_BaseMartini):
         torch = ins.torch
         arr = dc._array[..., 0] if dc.stokes_axis else dc._array
         nx, ny, nc = arr.shape
-        d_in = torch.from_numpy(np.ascontiguousarray(arr, np.float32)).to(ins.device)
+        resident = getattr(self, "_resident", None)
+        if resident is not None and resident[1] is dc._array:
+            d_in = resident[0].view(nx, ny, nc)  # the inserted cube, still on the device
+        else:
+            d_in = torch.from_numpy(np.ascontiguousarray(arr, np.float32)).to(ins.device)
+        self._resident = None
         d_out = torch.empty_like(d_in)
-        K = np.ascontiguousarray(self.beam.kernel, np.float32)
-        d_k = torch.from_numpy(K).to(ins.device)
-        _lib.convolve_beam(d_in, d_out, nx, ny, nc, d_k, K.shape[0], K.shape[1])
+        factors = beam_factors(self.beam.kernel)
+        if factors is not None:  # two one-dimensional passes
+            d_tmp = torch.empty_like(d_in)
+            for d_src, d_dst, f, (kx, ky) in ((d_in, d_tmp, factors[0], (factors[0].size, 1)),
+                                              (d_tmp, d_out, factors[1], (1, factors[1].size))):
+                d_k = torch.from_numpy(np.ascontiguousarray(f, np.float32)).to(ins.device)
+                _lib.convolve_beam(d_src, d_dst, nx, ny, nc, d_k, kx, ky)
+        else:
+            K = np.ascontiguousarray(self.beam.kernel, np.float32)
+            d_k = torch.from_numpy(K).to(ins.device)
+            _lib.convolve_beam(d_in, d_out, nx, ny, nc, d_k, K.shape[0], K.shape[1])
         out = d_out.cpu().numpy().astype(np.float64)
         dc._array = out[..., np.newaxis] if dc.stokes_axis else out
         dc.drop_pad()

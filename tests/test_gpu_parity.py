@@ -150,3 +150,24 @@ def test_frequency_channels(method):
     print(f"freq channels {method}: voxel {voxel:.3e} flux {flux:.3e}")
     assert voxel <= VOXEL_TOL
     assert flux <= FLUX_TOL
+
+
+@pytest.mark.parametrize("method", ["plain", "mfma"])
+def test_insert_twice_accumulates(method):
+    """insert_source_in_cube adds to the cube (martini/martini.py:338, `+=`):
+    a second insertion of the same source doubles it exactly (the same FP32
+    contribution added twice in FP64), and the Jy/pix^2 -> Jy/arcsec^2
+    conversion happens once."""
+    w = synthetic.WORKLOADS["disc1e4_64px"]
+    src, dc, k, s = synthetic.build(w, 3000)
+    from martini_amd import Martini
+
+    m = Martini(source=src, datacube=dc, sph_kernel=k, spectral_model=s, quiet=True)
+    m.insert_source_in_cube(method=method)
+    once = m.datacube._array.copy()
+    assert m.datacube._array_unit == "Jy/arcsec2"
+    m.insert_source_in_cube(method=method)
+    assert m.datacube._array_unit == "Jy/arcsec2"
+    assert once.max() > 0
+    assert np.array_equal(m.datacube._array, 2.0 * once)
+    assert m._resident is None  # the device cube no longer mirrors the host cube
