@@ -180,11 +180,12 @@ int mi_insert_noalloc(const mi_particle* d_particles, int64_t n, int64_t n_pairs
  * callers that time or overlap them.  mi_bin sorts the particles into cells
  * inside the workspace; mi_gather then evaluates the contraction over those
  * cells with `method` and adds it to d_cube.  mi_bin's `method` names the
- * gather it prepares for: the MFMA form gets each cell's particles in full
- * Z order (its skips are per 32-particle chunk), the FP32 form an order by a
- * hash of the particle index from a cheaper sort (each chunk then spreads over
- * the tile, which balances its products over the waves).  Either order gives
- * a correct gather with either method.  The workspace must not be reused between the
+ * gather it prepares for: the MFMA form gets each cell's particles in record
+ * order (records laid out by tile and pixel Z order make its 32-particle
+ * chunks compact, and its skips are per chunk), the FP32 form an order by a
+ * hash of the particle index (each chunk then spreads over the tile, which
+ * balances its products over the waves).  Either order gives a correct gather
+ * with either method.  The workspace must not be reused between the
  * two calls.  Overflow of n_pairs is reported by mi_status (which
  * synchronises the stream); a gather after an overflowed bin leaves the cube
  * untouched.

@@ -93,11 +93,6 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     return hipSuccess;
 }
 
-__device__ inline uint32_t spread4(uint32_t v) {
-    v &= 15u;
-    return (v & 1u) | ((v & 2u) << 1) | ((v & 4u) << 2) | ((v & 8u) << 3);
-}
-
 // Kernel families in a cell's sort order: tabulated closed forms first (the
 // item stages the first one's table in LDS), then Gaussian, Dirac, and the
 // caller's weight tables.
@@ -133,10 +128,12 @@ __global__ void k_count(const mi_particle* __restrict__ P, int64_t n, Grid g,
 }
 
 // Writes the (key, particle) pairs of each particle at its scanned offset.
-// The low 8 key bits are the pixel's Z order in the tile, or, with `mix`, a
-// hash of the particle index: the FP32 gather's waves each own 4 tile rows, and
-// a chunk of particles drawn from all over the tile gives them even shares of
-// the products, where a spatially compact chunk loads one or two of them.
+// With `mix` the low 8 key bits are a hash of the particle index: the FP32
+// gather's waves each own 4 tile rows, and a chunk of particles drawn from all
+// over the tile gives them even shares of the products, where a spatially
+// compact chunk loads one or two of them.  Without it they are zero and a
+// cell's pairs keep the record order (the MFMA form, whose records are laid
+// out in tile and pixel Z order).
 // Entries past the device total up to the capacity get key 0xffffffff so the
 // sort moves them to the end; a total above the capacity raises the overflow
 // flag and writes nothing.
@@ -161,16 +158,11 @@ __global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap
         int tx0, tx1, ty0, ty1, cb0, cb1;
         if (cells_of(p, g, tx0, tx1, ty0, ty1, cb0, cb1) == 0) continue;
         uint32_t at = offsets[k];
-        uint32_t kg = kernel_group(kind_of(p.r_kind));
+        const uint32_t kg = kernel_group(kind_of(p.r_kind));
         const uint32_t hash = ((uint32_t)k * 2654435761u) >> 24;
+        const uint32_t local = (kg << 8) | (mix ? hash : 0u);
         for (int tx = tx0; tx <= tx1; ++tx) {
-            int lx = p.ix - tx * TILE;
-            lx = lx < 0 ? 0 : (lx > TILE - 1 ? TILE - 1 : lx);
             for (int ty = ty0; ty <= ty1; ++ty) {
-                int ly = p.iy - ty * TILE;
-                ly = ly < 0 ? 0 : (ly > TILE - 1 ? TILE - 1 : ly);
-                uint32_t local =
-                    (kg << 8) | (mix ? hash : spread4((uint32_t)lx) | (spread4((uint32_t)ly) << 1));
                 for (int cb = cb0; cb <= cb1; ++cb) {
                     uint32_t cell = (uint32_t)(((int64_t)tx * g.nty + ty) * g.ncb + cb);
                     keys[at] = (cell << KEY_LOCAL_BITS) | local;
@@ -301,7 +293,7 @@ __global__ void k_item_keys(const Item* __restrict__ items, const uint32_t* __re
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
-                         const mi_cube_desc& d, char* ws, const Layout& L, bool full_order,
+                         const mi_cube_desc& d, char* ws, const Layout& L, bool record_order,
                          hipStream_t s) {
     uint32_t* counts = (uint32_t*)(ws + L.counts);
     uint32_t* offsets = (uint32_t*)(ws + L.offsets);
@@ -333,13 +325,11 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
     unsigned eb = blocks_for(emit_threads, T);
     if (eb > 65536u) eb = 65536u;
     hipLaunchKernelGGL(k_emit, dim3(eb), dim3(T), 0, s, d_p, n, n_pairs, g, counts, offsets, keys,
-                       vals, status, full_order ? 0 : 1);
+                       vals, status, record_order ? 0 : 1);
     if (n_pairs > 0) {
-        // The MFMA form wants compact chunks: it takes the two-pass sort too when
-        // that still orders at least the top 5 Z-order bits (2 x 4-pixel blocks,
-        // on grids of at most 2^9 cells), and all key bits otherwise.
-        const int cb = coarse_begin_bit(g);
-        const int begin_bit = full_order ? (cb <= 3 ? cb : 0) : cb;
+        // In record order only the cell and the kernel group are sorted (two
+        // radix passes up to 2^14 cells); the stable sort keeps the rest.
+        const int begin_bit = record_order ? KEY_LOCAL_BITS - 2 : coarse_begin_bit(g);
         hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys2, vals, vals2,
                                                  (size_t)n_pairs, begin_bit,
                                                  g.cell_bits + KEY_LOCAL_BITS, s);

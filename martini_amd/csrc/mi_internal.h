@@ -65,12 +65,13 @@ enum { ST_PAIRS = 0, ST_OVERFLOW = 1, ST_ITEMS = 2, ST_MULTI = 3, ST_BADREC = 4 
 hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L);
 
 // Binning: count, scan, emit, sort, cell bounds, items.  Asynchronous.
-// full_order: keys end in the pixel Z order and are sorted on every bit (the
-// MFMA form); otherwise they end in a hash of the particle index, which spreads
-// each chunk over its tile, and are sorted on the top 16 bits (two radix
-// passes) when the key is longer.
+// record_order: a cell's pairs keep the record order (the MFMA form: records
+// laid out in tile and pixel Z order make its chunks compact) and are sorted on
+// the cell and kernel group only; otherwise the keys end in a hash of the
+// particle index, which spreads each chunk over its tile, and are sorted on
+// the top 16 bits (two radix passes) when the key is longer.
 hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, const Grid& g,
-                         const mi_cube_desc& d, char* ws, const Layout& L, bool full_order,
+                         const mi_cube_desc& d, char* ws, const Layout& L, bool record_order,
                          hipStream_t s);
 
 // Gather over work items (plain FP32 or MFMA), then the partial-tile reduction.

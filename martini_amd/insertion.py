@@ -152,10 +152,10 @@ def _spread4(v):
 def spatial_records(d_rec):
     """Records (uint8 tensor of n * 48 bytes) sorted by 16 x 16-pixel tile of
     their pixel (ix, iy), tiles in the binning's cell order (x-major), and by
-    the pixel's Z order inside the tile: the order in which the MFMA form's
-    binning lists each cell's particles.  A chunk of 32 consecutive particles of
-    a cell then reads neighbouring records, whole cache lines instead of one
-    48-byte record per line.  Equal keys keep the given order (a stable sort),
+    the pixel's Z order inside the tile.  The MFMA form's binning keeps the
+    record order within each cell, so its 32-particle chunks are then compact
+    patches of the tile, and a chunk reads neighbouring records, whole cache
+    lines instead of one 48-byte record per line.  Equal keys keep the given order (a stable sort),
     so the layout is the same on every device."""
     import torch
 
