@@ -176,18 +176,79 @@ def test_dirac_spectrum_window_edge_inclusive():
     assert (c_lo[1], c_hi[1]) == (4, 4)
 
 
-def test_pair_count_matches_numpy():
-    w = synthetic.WORKLOADS["disc1e4_64px"]
-    m, rec, desc = _records(w)
+def _footprints(rec, nx, ny):
+    """The device's clipped footprint (mi_device.h footprint) in numpy: the
+    square mask of range r, cut for the tabulated kernels to the rows and
+    columns within h (1 + 2^-12) of the particle."""
     rr = rec["r_kind"] >> 8
-    ilo = np.maximum(rec["ix"] - rr + (rec["fx"] > 0), 0)
-    ihi = np.minimum(rec["ix"] + rr, desc.nx - 1)
-    jlo = np.maximum(rec["iy"] - rr + (rec["fy"] > 0), 0)
-    jhi = np.minimum(rec["iy"] + rr, desc.ny - 1)
+    ilo = rec["ix"] - rr + (rec["fx"] > 0)
+    ihi = rec["ix"] + rr
+    jlo = rec["iy"] - rr + (rec["fy"] > 0)
+    jhi = rec["iy"] + rr
+    h = rec["h"].astype(float)
+    cut = ((rec["r_kind"] & 15) <= _lib.KERNEL_QUARTIC) & (h > 0) & (h <= rr)
+    R = h * (1.0 + 2.0**-12)
+    x = rec["ix"] + rec["fx"].astype(float)
+    y = rec["iy"] + rec["fy"].astype(float)
+    ilo = np.where(cut, np.maximum(ilo, np.ceil(x - R)), ilo).astype(np.int64)
+    ihi = np.where(cut, np.minimum(ihi, np.floor(x + R)), ihi).astype(np.int64)
+    jlo = np.where(cut, np.maximum(jlo, np.ceil(y - R)), jlo).astype(np.int64)
+    jhi = np.where(cut, np.minimum(jhi, np.floor(y + R)), jhi).astype(np.int64)
+    ilo, jlo = np.maximum(ilo, 0), np.maximum(jlo, 0)
+    ihi, jhi = np.minimum(ihi, nx - 1), np.minimum(jhi, ny - 1)
+    return rr, ilo, ihi, jlo, jhi, cut
+
+
+@pytest.mark.parametrize("name", ["disc1e4_64px", "hetero3e6_1024px"])
+def test_pair_count_matches_numpy(name):
+    w = synthetic.WORKLOADS[name]
+    m, rec, desc = _records(w, None if w.n <= 10_000 else 30_000)
+    rr, ilo, ihi, jlo, jhi, cut = _footprints(rec, desc.nx, desc.ny)
+    assert cut.mean() > 0.5  # the cut applies to most particles here
     clo, chi = np.maximum(rec["c_lo"], 0), np.minimum(rec["c_hi"], desc.nc - 1)
     ok = (rr >= 0) & (ilo <= ihi) & (jlo <= jhi) & (clo <= chi)
     n = ((ihi // 16 - ilo // 16 + 1) * (jhi // 16 - jlo // 16 + 1) * (chi // 64 - clo // 64 + 1))
     assert _lib.count_pairs(rec, desc) == int(n[ok].sum())
+
+
+@pytest.mark.parametrize("kernel", ["cubic", "wc2", "wc6", "quartic"])
+def test_cut_footprint_drops_only_zero_weights(kernel):
+    """Every pixel of the square mask that the device's footprint leaves out
+    has weight exactly zero in the oracle's kernel integral, for h drawn up to
+    the mask's range; and the footprint keeps pixels at q just below 1."""
+    spec = O.KernelSpec(kernel)
+    rng = np.random.default_rng(5)
+    n = 4000
+    rr = rng.integers(1, 9, n)
+    h = (rr * rng.uniform(0.3, 1.0, n)).astype(np.float32)
+    h[:8] = rr[:8]  # h = r exactly: the cut meets the mask
+    x = rng.uniform(10, 20, n)
+    y = rng.uniform(10, 20, n)
+    rec = np.zeros(n, _lib.PARTICLE_DTYPE)
+    rec["ix"], rec["iy"] = np.floor(x), np.floor(y)
+    rec["fx"] = (x - np.floor(x)).astype(np.float32)
+    rec["fy"] = (y - np.floor(y)).astype(np.float32)
+    rec["h"] = h
+    rec["r_kind"] = (rr << 8) | _lib.KERNEL_CUBIC
+    x = rec["ix"] + rec["fx"].astype(float)
+    y = rec["iy"] + rec["fy"].astype(float)
+    _, ilo, ihi, jlo, jhi, cut = _footprints(rec, 64, 64)
+    assert cut.all()
+    dropped = kept_edge = 0
+    for p in range(n):
+        I = np.arange(np.ceil(x[p] - rr[p]), np.floor(x[p] + rr[p]) + 1)[:, None]
+        J = np.arange(np.ceil(y[p] - rr[p]), np.floor(y[p] + rr[p]) + 1)[None, :]
+        I, J = I + 0 * J, J + 0 * I  # the square mask, |i - x| <= r and |j - y| <= r
+        kept = (I >= ilo[p]) & (I <= ihi[p]) & (J >= jlo[p]) & (J <= jhi[p])
+        out = ~kept
+        wts = spec.integral(x[p] - I[out], y[p] - J[out], float(h[p]))
+        assert np.all(wts == 0.0), (p, wts.max())
+        dropped += int(out.sum())
+        q = np.hypot(x[p] - I, y[p] - J) / float(h[p])
+        edge = (q < 1) & (q > 0.97)
+        assert np.all(kept[edge])
+        kept_edge += int(edge.sum())
+    assert dropped > 10 * n and kept_edge > n
 
 
 def test_library_exports_every_declared_symbol():
