@@ -44,7 +44,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // store, which then spreads over the LDS banks
     __shared__ __attribute__((aligned(16))) float St[KC][CBLOCK + 4];
     __shared__ ChunkState cs;
-    __shared__ float4 Tk[WTAB_N];  // the item's kernel table
+    __shared__ float4 Tk[LTAB_N];  // the item's kernel table
     __shared__ GaussTables gt;     // the chunk's Gaussian-kernel factors
 
     const int t = threadIdx.x;
@@ -112,7 +112,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             fill_gauss_tables(cs, gt);
             __syncthreads();
         }
-        fill_weights(cs, x0, y0, wtab, tkind, Tk, gt, d.weights, [&](int k, int px, float w) { Wt[k][px] = w; });
+        fill_weights(
+            cs, x0, y0, wtab, tkind, Tk, gt, d.weights, [&](int k, int px, float w) { Wt[k][px] = w; },
+            [&](int k, int p1, float w1, int p2, float w2) {
+                Wt[k][p1] = w1;
+                Wt[k][p2] = w2;
+            });
         fill_spectra(cs, nk, c0, g.nc, d, [&](int k, int cl, float v) { St[k][cl] = v; });
         __syncthreads();
         if (kmask == 0u) continue;

@@ -35,22 +35,41 @@
 namespace mi {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 constexpr int KROW = KC * 2;  // bytes per LDS operand row (32 BF16)
 
-// Byte offset of element (row, k) in an operand tile.
-__device__ inline int op_offset(int row, int k) {
-    return row * KROW + ((((k >> 3) ^ (row >> 2)) & 3) << 4) + ((k & 7) << 1);
-}
+// Byte offset of element (row, k) in an operand tile:
+//   row * KROW + (((k >> 3) ^ (row >> 2)) & 3) * 16 + (k & 7) * 2,
+// written with the k terms folded into one constant per thread (k_bits(k)).
+__device__ inline int k_bits(int k) { return (((k >> 3) & 3) << 4) | ((k & 7) << 1); }
+__device__ inline int op_offset(int row, int kb) { return row * KROW + (((row << 2) & 48) ^ kb); }
 
 // Writes x as a BF16 pair (round to nearest even) at (row, k) of the hi and lo tiles.
 __device__ inline void put_split(char* hi, char* lo, int row, int k, float x) {
     const __bf16 h = (__bf16)x;
     const __bf16 l = (__bf16)(x - (float)h);
-    const int o = op_offset(row, k);
+    const int o = op_offset(row, k_bits(k));
     *reinterpret_cast<__bf16*>(hi + o) = h;
     *reinterpret_cast<__bf16*>(lo + o) = l;
+}
+
+// put_split of two values of column k, converted together (one packed
+// conversion each for the hi and the lo parts).
+__device__ inline void put_split2(char* hi, char* lo, int k, int row1, float x1, int row2,
+                                  float x2) {
+    const floatx2 x = {x1, x2};
+    const bf16x2 h = __builtin_convertvector(x, bf16x2);
+    const floatx2 hf = __builtin_convertvector(h, floatx2);
+    const bf16x2 l = __builtin_convertvector(floatx2{x1 - hf.x, x2 - hf.y}, bf16x2);
+    const int kb = k_bits(k);
+    const int o1 = op_offset(row1, kb), o2 = op_offset(row2, kb);
+    *reinterpret_cast<__bf16*>(hi + o1) = h.x;
+    *reinterpret_cast<__bf16*>(lo + o1) = l.x;
+    *reinterpret_cast<__bf16*>(hi + o2) = h.y;
+    *reinterpret_cast<__bf16*>(lo + o2) = l.y;
 }
 
 // The 8 BF16 values of k-group kg of `row`.
@@ -76,7 +95,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ __attribute__((aligned(16))) char Bh[CBLOCK * KROW];   // S hi [channel][k]
     __shared__ __attribute__((aligned(16))) char Bl[CBLOCK * KROW];   // S lo
     __shared__ ChunkState cs;
-    __shared__ float4 Tk[WTAB_N];  // the item's kernel table
+    __shared__ float4 Tk[LTAB_N];  // the item's kernel table
     __shared__ GaussTables gt;     // the chunk's Gaussian-kernel factors
 
     const int t = threadIdx.x;
@@ -130,7 +149,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             __syncthreads();
         }
         fill_weights(cs, x0, y0, wtab, tkind, Tk, gt, d.weights,
-                     [&](int k, int px, float w) { put_split(Ah, Al, px, k, w); });
+                     [&](int k, int px, float w) { put_split(Ah, Al, px, k, w); },
+                     [&](int k, int p1, float w1, int p2, float w2) {
+                         put_split2(Ah, Al, k, p1, w1, p2, w2);
+                     });
         fill_spectra(cs, nk, c0, g.nc, d,
                      [&](int k, int cl, float s) { put_split(Bh, Bl, cl, k, s); }, chans);
         __syncthreads();

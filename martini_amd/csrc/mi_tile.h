@@ -21,6 +21,9 @@ namespace mi {
 
 constexpr int KC = 32;    // particles per chunk
 constexpr int NT = 256;   // threads per work group
+// An item's kernel table in LDS: the WTAB_N intervals and one zero entry
+// after them, which the branch-free walk reads for q >= 1.
+constexpr int LTAB_N = WTAB_N + 1;
 
 // What the W walk needs of a particle, read with two 16-byte loads whenever
 // a thread moves on to the next particle.
@@ -214,24 +217,24 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
 }
 
 // The weight of a tabulated kernel at q^2 = dr^2 / h^2, as weight_at computes
-// it, without branches: fb = 1/h^2, tab the kernel's table, CUBIC whether the
-// seam term of the cubic spline applies.  q is clamped to the table's range,
-// so the row is read for every pixel (no branch around the LDS load, and the
-// loads of neighbouring evaluations overlap), and the weight is masked to zero
-// for q >= 1 by the factor.
+// it, without branches: fb = 1/h^2, tab the kernel's table with its zero
+// entry at LTAB_N - 1, CUBIC whether the seam term of the cubic spline
+// applies.  The table position is clamped to that zero entry, so q >= 1 reads
+// it (and the seam term is zero beyond q = 1/2): no mask, and the row is read
+// for every pixel (no branch around the LDS load).
 template <bool CUBIC>
 __device__ inline float table_weight_select(const float4* __restrict__ tab, float q2, float fb) {
     const float q = __builtin_amdgcn_sqrtf(q2);
-    const float x = fminf(q, 1.0f) * (float)WTAB_N;
-    const int i = min((int)x, WTAB_N - 1);
-    const float t = x - (float)i;
+    const float x = fminf(q * (float)WTAB_N, (float)WTAB_N);
+    const int i = (int)x;
+    const float t = __builtin_amdgcn_fractf(x);
     const float4 c = tab[i];
     float w = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
     if (CUBIC) {  // R2 sqrt(1 - R2) / (8 * 1.5969), R2 = 4 q^2, zero for R2 >= 1
         w = fmaf(q2 * (float)(4.0 / (8.0 * 1.59689476201133)),
                  __builtin_amdgcn_sqrtf(fmaxf(fmaf(-4.0f, q2, 1.0f), 0.0f)), w);
     }
-    return w * (q < 1.0f ? fb : 0.0f);
+    return w * fb;
 }
 
 // Fills gt for the chunk's Gaussian particles (cs.gmask): thread t takes
@@ -264,17 +267,18 @@ __device__ inline float gauss_weight(double dx, double dy, double fa, double fe,
 }
 
 // Evaluates W for every (particle, pixel) pair of the chunk and hands each
-// value to store(k, pixel-in-tile, w).  Particle k's pairs (pixel row l / width,
+// value to store(k, pixel-in-tile, w), or two of them to
+// store2(k, pixel1, w1, pixel2, w2).  Particle k's pairs (pixel row l / width,
 // column l % width of its clipped footprint, l < cnt) are split over threads
 // toff[k] .. toff[k+1] - 1, `per` consecutive pairs each, so a thread stays on
 // one particle: one binary search, then a walk over its pixels.  When every
 // particle of the wave's pairs uses the item's LDS table, the walk is
 // branch-free and takes two pixels per step, whose evaluations overlap.
-template <class Store>
+template <class Store, class Store2>
 __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                                     const float4* __restrict__ wtab, int tab_kind,
                                     const float4* __restrict__ tab, const GaussTables& gt,
-                                    const float* __restrict__ uw, Store store) {
+                                    const float* __restrict__ uw, Store store, Store2 store2) {
     const int t = threadIdx.x;
     const bool active = t < cs.toff[KC];
     int k = 0;
@@ -313,8 +317,7 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                 b = b + 1 == width ? 0 : b + 1;
                 a = b == 0 ? a + 1 : a;
                 const float v1 = eval(a1, b1), v2 = eval(a2, b2);
-                store(k, (ti + a1) * TILE + tj + b1, v1);
-                store(k, (ti + a2) * TILE + tj + b2, v2);
+                store2(k, (ti + a1) * TILE + tj + b1, v1, (ti + a2) * TILE + tj + b2, v2);
             }
             if (l < l1) store(k, (ti + a) * TILE + tj + b, eval(a, b));
         };
@@ -443,7 +446,8 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
 }
 
 // Copies the table of the kernel kind of the item's first particle (when it
-// is a tabulated kind) into LDS; returns that kind, or -1.  Particles of one
+// is a tabulated kind) into LDS, with a zero entry after it; returns that
+// kind, or -1.  Particles of one
 // cell are sorted with the tabulated kinds first, so this is the table the
 // item uses (a second tabulated kind in the same cell reads the global copy).
 // The copy is complete after the next __syncthreads().
@@ -453,6 +457,7 @@ __device__ inline int stage_table(const mi_particle* __restrict__ P,
     const int kind = kind_of(P[order[begin]].r_kind);
     if (kind > MI_KERNEL_QUARTIC) return -1;
     for (int e = threadIdx.x; e < WTAB_N; e += NT) local[e] = wtab[kind * WTAB_N + e];
+    if (threadIdx.x == 0) local[WTAB_N] = make_float4(0.f, 0.f, 0.f, 0.f);
     return kind;
 }
 
