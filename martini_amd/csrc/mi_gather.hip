@@ -118,7 +118,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 Wt[k][p1] = w1;
                 Wt[k][p2] = w2;
             });
-        fill_spectra(cs, nk, c0, g.nc, d, [&](int k, int cl, float v) { St[k][cl] = v; });
+        fill_spectra(cs, nk, c0, g.nc, d, [&](int k, int c1, float v1, int c2, float v2) {
+            St[k][c1] = v1;
+            St[k][c2] = v2;
+        });
         __syncthreads();
         if (kmask == 0u) continue;
         while (kmask != 0u) {

@@ -154,7 +154,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                          put_split2(Ah, Al, k, p1, w1, p2, w2);
                      });
         fill_spectra(cs, nk, c0, g.nc, d,
-                     [&](int k, int cl, float s) { put_split(Bh, Bl, cl, k, s); }, chans);
+                     [&](int k, int c1, float s1, int c2, float s2) {
+                         put_split2(Bh, Bl, k, c1, s1, c2, s2);
+                     },
+                     chans);
         __syncthreads();
         if (rows != 0u) {
 #pragma unroll

@@ -370,17 +370,21 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
 }
 
 // Evaluates S for the chunk's KC particles x CBLOCK channels of the cell and
-// hands each value to store(k, channel-in-block, s); particles k >= nk and
-// channels beyond the cube give 0.  Thread t takes particle t % KC and the
-// channels t / KC + 8 i, so its record fields are read once.  Only the
-// 16-channel blocks set in `blocks` (wave-uniform) are evaluated; the others
-// are left untouched.  The values are line_flux's (for narrow spans the same
-// series with its polynomial expanded, to FP32 rounding), computed as
-// straight-line code (the window test is a select, the wide-channel erfc path
-// one branch per block) so that the evaluations of a thread overlap.
-template <class Store>
+// hands the values to store2(k, channel1, s1, channel2, s2), two channels of
+// one particle at a time; particles k >= nk and channels beyond the cube give
+// 0.  Thread t takes particle t % KC and the channels t / KC + 8 i, so its
+// record fields are read once.  Only the 16-channel blocks set in `blocks`
+// (wave-uniform) are evaluated; the others are left untouched.  The values
+// are line_flux's (for narrow spans the same series with its polynomial
+// expanded, to FP32 rounding), computed as straight-line code (the window
+// test is one unsigned compare and a select, the wide-channel erfc path one
+// branch per block) so that the evaluations of a thread overlap.
+template <class Store2>
 __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc,
-                                    const mi_cube_desc& d, Store store, uint32_t blocks = 0xFu) {
+                                    const mi_cube_desc& d, Store2 store2,
+                                    uint32_t blocks = 0xFu) {
+    constexpr int PER_BLOCK = 16 / (NT / KC);  // a thread's channels in one 16-channel block
+    static_assert(PER_BLOCK == 2, "a thread stores its two channels of a block together");
     const int k = threadIdx.x & (KC - 1);
     const int cl0 = threadIdx.x / KC;
     const mi_particle& p = cs.p[k];
@@ -389,18 +393,31 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     const int k0 = p.k0;
     const float fu = p.fu, s = p.s, amp = p.amp;
     const int spectrum = d.spectrum;
-    const bool dirac = spectrum == MI_SPECTRUM_DIRAC;
-    constexpr int PER_BLOCK = 16 / (NT / KC);  // a thread's channels in one 16-channel block
+    // Channel cl0 + o of the cell lies in the window [lo, hi] when
+    // (unsigned)(o - w0) <= wl; an empty window takes an offset no o reaches.
+    const int w0 = lo <= hi ? lo - c0 - cl0 : (1 << 30);
+    const uint32_t wl = lo <= hi ? (uint32_t)(hi - lo) : 0u;
+    auto inside = [&](int o) { return (uint32_t)(o - w0) <= wl; };
+    auto put = [&](int b, float v0, float v1) {
+        constexpr int STEP = NT / KC;
+        const int o0 = STEP * (b * PER_BLOCK), o1 = o0 + STEP;
+        store2(k, cl0 + o0, inside(o0) ? v0 : 0.0f, cl0 + o1, inside(o1) ? v1 : 0.0f);
+    };
     if (spectrum == MI_SPECTRUM_TABLE) {  // uniform: the caller's spectra
 #pragma unroll
         for (int b = 0; b < CBLOCK / 16; ++b) {
             if (!((blocks >> b) & 1u)) continue;
-#pragma unroll
-            for (int i = 0; i < PER_BLOCK; ++i) {
-                const int cl = cl0 + (NT / KC) * (b * PER_BLOCK + i);
-                store(k, cl, table_flux(lo, hi, k0, amp, c0 + cl, d.spectra, d.spectra_ld));
-            }
+            const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK);
+            store2(k, c - c0, table_flux(lo, hi, k0, amp, c, d.spectra, d.spectra_ld),
+                   c - c0 + NT / KC,
+                   table_flux(lo, hi, k0, amp, c + NT / KC, d.spectra, d.spectra_ld));
         }
+        return;
+    }
+    if (spectrum == MI_SPECTRUM_DIRAC) {  // uniform: the whole amplitude in the window
+#pragma unroll
+        for (int b = 0; b < CBLOCK / 16; ++b)
+            if ((blocks >> b) & 1u) put(b, amp, amp);
         return;
     }
     // Narrow spans: span_series with its D-dependent factors folded once per
@@ -424,24 +441,19 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
             const float m = s * (A - (float)((NT / KC) * (b * PER_BLOCK + i))), m2 = m * m;
             v[i] = __expf(-m2) * fmaf(fmaf(fmaf(e3, m2, e2), m2, e1), m2, e0);
         }
-        if (!dirac && s > 0.5f) {
+        if (s > 0.5f) {
             // wide channels: a line this narrow spans a few channels, and the
             // erfc differences are taken only inside its window
 #pragma unroll
             for (int i = 0; i < PER_BLOCK; ++i) {
-                const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK + i);
-                if (c < lo || c > hi) continue;
+                const int o = (NT / KC) * (b * PER_BLOCK + i);
+                if (!inside(o)) continue;
+                const int c = c0 + cl0 + o;
                 const float m = s * (((float)(k0 - c) + fu) - 0.5f);
                 v[i] = 0.5f * amp * erf_diff(m - 0.5f * s, m + 0.5f * s);
             }
         }
-#pragma unroll
-        for (int i = 0; i < PER_BLOCK; ++i) {
-            const int cl = cl0 + (NT / KC) * (b * PER_BLOCK + i);
-            const int c = c0 + cl;
-            const float f = dirac ? amp : v[i];
-            store(k, cl, (c >= lo && c <= hi) ? f : 0.0f);
-        }
+        put(b, v[0], v[1]);
     }
 }
 
