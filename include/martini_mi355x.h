@@ -140,7 +140,9 @@ const char* mi_strerror(int code);
 int64_t mi_count_pairs_host(const mi_particle* h_particles, int64_t n,
                             const mi_cube_desc* desc);
 
-/* Workspace bytes for mi_insert with n particles producing n_pairs pairs. */
+/* Workspace bytes for mi_insert with n particles producing n_pairs pairs.
+ * Every entry point that takes d_workspace requires it 256-byte aligned (as
+ * hipMalloc returns it) and refuses it otherwise with MI_ERR_ARGUMENT. */
 int mi_insert_workspace_size(const mi_cube_desc* desc, int64_t n, int64_t n_pairs,
                              size_t* bytes);
 

@@ -186,16 +186,32 @@ __global__ void k_bin_init(uint32_t* __restrict__ status, uint32_t* __restrict__
     }
 }
 
+// Cell bounds from the sorted keys: thread t takes keys 4t .. 4t + 3 (one
+// 16-byte load when the run lies inside the array, the workspace keys being
+// 256-byte aligned) and the key on either side.
 __global__ void k_cell_bounds(const uint32_t* __restrict__ keys, int64_t np, int64_t ncells,
                               uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
-    uint32_t c = keys[q] >> KEY_LOCAL_BITS;
-    if ((int64_t)c >= ncells) return;  // padding
-    uint32_t prev = q > 0 ? keys[q - 1] >> KEY_LOCAL_BITS : 0xffffffffu;
-    uint32_t next = q + 1 < np ? keys[q + 1] >> KEY_LOCAL_BITS : 0xffffffffu;
-    if (c != prev) start[c] = (uint32_t)q;
-    if (c != next) end[c] = (uint32_t)(q + 1);
+    const int64_t q0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (q0 >= np) return;
+    uint32_t c[6];  // cells of keys q0 - 1 .. q0 + 4 (0xffffffff outside the array)
+    if (q0 + 4 <= np) {
+        const uint4 v = *reinterpret_cast<const uint4*>(keys + q0);
+        c[1] = v.x >> KEY_LOCAL_BITS;
+        c[2] = v.y >> KEY_LOCAL_BITS;
+        c[3] = v.z >> KEY_LOCAL_BITS;
+        c[4] = v.w >> KEY_LOCAL_BITS;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i + 1] = q0 + i < np ? keys[q0 + i] >> KEY_LOCAL_BITS : 0xffffffffu;
+    }
+    c[0] = q0 > 0 ? keys[q0 - 1] >> KEY_LOCAL_BITS : 0xffffffffu;
+    c[5] = q0 + 4 < np ? keys[q0 + 4] >> KEY_LOCAL_BITS : 0xffffffffu;
+#pragma unroll
+    for (int i = 1; i <= 4; ++i) {
+        if ((int64_t)c[i] >= ncells) continue;  // padding (and past the end)
+        if (c[i] != c[i - 1]) start[c[i]] = (uint32_t)(q0 + i - 1);
+        if (c[i] != c[i + 1]) end[c[i]] = (uint32_t)(q0 + i);
+    }
 }
 
 // Work items per cell (ceil(len / item_len), none after an overflow) and
@@ -336,8 +352,8 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
         if (e != hipSuccess) return e;
     }
     if (n_pairs > 0)
-        hipLaunchKernelGGL(k_cell_bounds, dim3(blocks_for(n_pairs, T)), dim3(T), 0, s, keys2,
-                           n_pairs, g.ncells, cstart, cend);
+        hipLaunchKernelGGL(k_cell_bounds, dim3(blocks_for((n_pairs + 3) / 4, T)), dim3(T), 0, s,
+                           keys2, n_pairs, g.ncells, cstart, cend);
     hipLaunchKernelGGL(k_item_scan, dim3(1), dim3(SCAN_T), 0, s, cstart, cend, g.ncells,
                        L.item_len, status, ioff, moff);
     hipLaunchKernelGGL(k_build_items, dim3((unsigned)g.ncells), dim3(64), 0, s, cstart, cend,

@@ -104,6 +104,7 @@ static int prepare(const mi_particle* d_particles, int64_t n, int64_t n_pairs,
                    Grid* g, Layout* L) {
     if (!desc_ok(desc) || n < 0 || n_pairs < 0 || !d_workspace) return MI_ERR_ARGUMENT;
     if (n > 0 && !d_particles) return MI_ERR_ARGUMENT;
+    if (reinterpret_cast<uintptr_t>(d_workspace) & 255u) return MI_ERR_ARGUMENT;
     *g = make_grid(*desc);
     int rc = check_limits(*g, n, n_pairs);
     if (rc) return rc;
@@ -150,6 +151,7 @@ int mi_status(const mi_cube_desc* desc, int64_t n, int64_t n_pairs, const void* 
     Grid g;
     Layout L;
     if (!desc_ok(desc) || !d_workspace || n < 0 || n_pairs < 0) return MI_ERR_ARGUMENT;
+    if (reinterpret_cast<uintptr_t>(d_workspace) & 255u) return MI_ERR_ARGUMENT;
     g = make_grid(*desc);
     if (layout_for(g, n, n_pairs, &L) != hipSuccess) return MI_ERR_DEVICE;
     uint32_t st[8];
@@ -245,6 +247,7 @@ int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
     if (!observation_ok(obs) || n < 0 || !d_counts) return MI_ERR_ARGUMENT;
     if (n > 0 && (!d_xyz || !d_vxyz || !d_mHI || !d_hsm || !d_out || !d_workspace))
         return MI_ERR_ARGUMENT;
+    if (reinterpret_cast<uintptr_t>(d_workspace) & 255u) return MI_ERR_ARGUMENT;
     if (n > 0 && obs->spectrum == MI_SPECTRUM_GAUSS && obs->sigma_kms < 0.0 && !d_T)
         return MI_ERR_ARGUMENT;
     if (n > 0 && obs->spectrum == MI_SPECTRUM_DIRAC && !d_edges) return MI_ERR_ARGUMENT;

@@ -274,6 +274,10 @@ def test_abi_argument_errors():
     assert L.mi_kernel_weights(9, 0.0, 1.0, None, None, None, None, 0, None) == -4
     d = _lib.make_desc(4, 4, 4, 0)
     assert L.mi_insert(None, 0, 0, ctypes.byref(d), None, None, 0, 0, None) == -1
+    # a workspace not 256-byte aligned is refused before any device work
+    for entry in (L.mi_bin, L.mi_gather):
+        args = (None, 0, 0, ctypes.byref(d)) + ((None,) if entry is L.mi_gather else ())
+        assert entry(*args, ctypes.c_void_p(0x10010), ctypes.c_size_t(1 << 20), 0, None) == -1
     assert L.mi_strerror(-2) == b"workspace too small for this input"
 
 
