@@ -11,10 +11,10 @@
 // such cells get one partial tile per item.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include "mi_device.h"
 #include "mi_internal.h"
+#include "mi_tile.h"
 
 namespace mi {
 
@@ -31,6 +31,16 @@ Grid make_grid(const mi_cube_desc& d) {
     while (((int64_t)1 << b) < g.ncells) ++b;
     g.cell_bits = b;
     return g;
+}
+
+// Count and emit passes: EMIT_T threads per block, EMIT_ROUNDS particles per
+// thread.
+constexpr int EMIT_T = 256;
+constexpr int EMIT_ROUNDS = 4;
+constexpr int64_t EMIT_SPAN = (int64_t)EMIT_T * EMIT_ROUNDS;
+
+__host__ __device__ inline int64_t emit_blocks(int64_t n) {
+    return n > 0 ? (n + EMIT_SPAN - 1) / EMIT_SPAN : 0;
 }
 
 // Lowest key bit the coarse sort orders on: the top 16 bits of the key
@@ -50,13 +60,14 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
         off = align256(off + bytes);
         return at;
     };
-    size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)(n_pairs > 0 ? n_pairs : 1);
+    size_t np = (size_t)(n_pairs > 0 ? n_pairs : 1);
+    const size_t nb2 = (size_t)emit_blocks(n) + 2;
     size_t nc1 = (size_t)g.ncells + 1;
     L->item_len = item_length(n_pairs);
     L->items_max = g.ncells + n_pairs / L->item_len + 1;
     L->multi_max = 2 * (n_pairs / L->item_len) + 1;
-    L->counts = take(4 * nn);
-    L->offsets = take(4 * nn);
+    L->counts = take(4 * nb2);
+    L->offsets = take(4 * nb2);
     L->keys = take(4 * np);
     L->vals = take(4 * np);
     L->keys2 = take(4 * np);
@@ -72,21 +83,16 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     L->item_order = take(4 * (size_t)L->items_max);
     L->partials = take(sizeof(float) * (size_t)L->multi_max * TILE_PX * CBLOCK);
     L->status = take(4 * 8);
-    // rocPRIM temporary storage: the larger of its two uses.
+    // rocPRIM temporary storage: the larger of its two uses (the pair and the item sort).
     size_t t1 = 0, t2 = 0;
     uint32_t* nul = nullptr;
-    hipError_t e = rocprim::exclusive_scan(nullptr, t1, nul, nul, 0u, nn,
-                                           rocprim::plus<uint32_t>(), 0);
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, t1, nul, nul, nul, nul, np, 0,
+                                             g.cell_bits + KEY_LOCAL_BITS, 0);
     if (e != hipSuccess) return e;
-    e = rocprim::radix_sort_pairs(nullptr, t2, nul, nul, nul, nul, np, 0,
-                                  g.cell_bits + KEY_LOCAL_BITS, 0);
-    if (e != hipSuccess) return e;
-    size_t t3 = 0;
-    e = rocprim::radix_sort_pairs(nullptr, t3, nul, nul, nul, nul, (size_t)L->items_max, 0,
+    e = rocprim::radix_sort_pairs(nullptr, t2, nul, nul, nul, nul, (size_t)L->items_max, 0,
                                   ITEM_KEY_BITS, 0);
     if (e != hipSuccess) return e;
-    size_t tb = t1 > t2 ? t1 : t2;
-    tb = tb > t3 ? tb : t3;
+    const size_t tb = t1 > t2 ? t1 : t2;
     L->temp_bytes = tb;
     L->temp = take(tb);
     L->total = off;
@@ -115,16 +121,83 @@ __device__ inline bool record_ok(const mi_particle& p, const Grid& g, const floa
     return end <= uw_n;
 }
 
-__global__ void k_count(const mi_particle* __restrict__ P, int64_t n, Grid g,
-                        const float* __restrict__ uw, int64_t uw_n,
-                        uint32_t* __restrict__ counts, uint32_t* __restrict__ status) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    mi_particle p = P[k];
-    int a, b, c, d, e, f;
-    const uint32_t cells = cells_of(p, g, a, b, c, d, e, f);
-    counts[k] = cells;
-    if (cells > 0 && !record_ok(p, g, uw, uw_n)) status[ST_BADREC] = 1u;  // same value from all
+// Count and emit passes: block b takes particles b * EMIT_SPAN + i * EMIT_T + t
+// (rounds i = 0 .. EMIT_ROUNDS - 1, thread t), so the record loads of a round
+// are contiguous.  k_count writes each block's pair total, k_scan_blocks turns
+// them into block bases, and k_emit recounts its particles and scans them in
+// the block: particle k's pairs start at the sum of the counts before k, as
+// one global exclusive scan would place them, without a per-particle array.
+
+// Exclusive block-wide scan (EMIT_T threads, all active); the block total in `total`.
+__device__ inline uint32_t block_excl_scan(uint32_t x, uint32_t* wsum, uint32_t& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t incl = (uint32_t)wave_scan_add((int)x);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < EMIT_T / 64; ++i) {
+        const uint32_t v = wsum[i];
+        base += i < w ? v : 0u;
+        tot += v;
+    }
+    __syncthreads();  // wsum may be written again
+    total = tot;
+    return base + incl - x;
+}
+
+__global__ __launch_bounds__(EMIT_T) void k_count(const mi_particle* __restrict__ P, int64_t n,
+                                                  Grid g, const float* __restrict__ uw,
+                                                  int64_t uw_n, uint32_t* __restrict__ block_tot,
+                                                  uint32_t* __restrict__ status) {
+    __shared__ uint32_t wsum[EMIT_T / 64];
+    const int64_t base = (int64_t)blockIdx.x * EMIT_SPAN + threadIdx.x;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < EMIT_ROUNDS; ++i) {
+        const int64_t k = base + (int64_t)i * EMIT_T;
+        if (k >= n) break;
+        const mi_particle p = P[k];
+        int a, b, c, d, e, f;
+        const uint32_t cells = cells_of(p, g, a, b, c, d, e, f);
+        sum += cells;
+        if (cells > 0 && !record_ok(p, g, uw, uw_n)) status[ST_BADREC] = 1u;  // same value from all
+    }
+    uint32_t total;
+    block_excl_scan(sum, wsum, total);
+    if (threadIdx.x == 0) block_tot[blockIdx.x] = total;
+}
+
+// Exclusive scan of the nb block totals in one work group (runs per thread,
+// then the run totals in LDS); base[nb], base[nb + 1] hold the 64-bit grand
+// total (low, high word), so a device count beyond 2^32 is still seen.
+constexpr int SCAN_T = 1024;
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_blocks(const uint32_t* __restrict__ tot,
+                                                        int64_t nb, uint32_t* __restrict__ base) {
+    __shared__ unsigned long long s[SCAN_T];
+    const int t = threadIdx.x;
+    const int64_t per = (nb + SCAN_T - 1) / SCAN_T;
+    const int64_t b0 = min((int64_t)t * per, nb), b1 = min(b0 + per, nb);
+    unsigned long long run = 0;
+    for (int64_t b = b0; b < b1; ++b) run += tot[b];
+    s[t] = run;
+    __syncthreads();
+    for (int d = 1; d < SCAN_T; d <<= 1) {  // inclusive scan of the run totals
+        const unsigned long long v = t >= d ? s[t - d] : 0ull;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    unsigned long long at = s[t] - run;
+    for (int64_t b = b0; b < b1; ++b) {
+        base[b] = (uint32_t)at;
+        at += tot[b];
+    }
+    if (t == SCAN_T - 1) {
+        base[nb] = (uint32_t)s[t];
+        base[nb + 1] = (uint32_t)(s[t] >> 32);
+    }
 }
 
 // Writes the (key, particle) pairs of each particle at its scanned offset.
@@ -137,27 +210,43 @@ __global__ void k_count(const mi_particle* __restrict__ P, int64_t n, Grid g,
 // Entries past the device total up to the capacity get key 0xffffffff so the
 // sort moves them to the end; a total above the capacity raises the overflow
 // flag and writes nothing.
-__global__ void k_emit(const mi_particle* __restrict__ P, int64_t n, int64_t cap, Grid g,
-                       const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
-                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                       uint32_t* __restrict__ status, int mix) {
+__global__ __launch_bounds__(EMIT_T) void k_emit(const mi_particle* __restrict__ P, int64_t n,
+                                                 int64_t cap, Grid g,
+                                                 const uint32_t* __restrict__ block_base,
+                                                 uint32_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals,
+                                                 uint32_t* __restrict__ status, int mix) {
+    __shared__ uint32_t wsum[EMIT_T / 64];
+    const int64_t nb = emit_blocks(n);
+    const uint64_t total = (uint64_t)block_base[nb] | ((uint64_t)block_base[nb + 1] << 32);
     const int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    uint64_t total = n > 0 ? (uint64_t)offsets[n - 1] + counts[n - 1] : 0;
     if (k0 == 0) {
         status[ST_PAIRS] = (uint32_t)total;
         status[ST_OVERFLOW] = total > (uint64_t)cap ? 1u : 0u;
     }
     if (total > (uint64_t)cap) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t q = (int64_t)total + k0; q < cap; q += stride) {  // padding entries
         keys[q] = 0xffffffffu;
         vals[q] = 0u;
     }
-    for (int64_t k = k0; k < n; k += stride) {
-        mi_particle p = P[k];
-        int tx0, tx1, ty0, ty1, cb0, cb1;
-        if (cells_of(p, g, tx0, tx1, ty0, ty1, cb0, cb1) == 0) continue;
-        uint32_t at = offsets[k];
+    if ((int64_t)blockIdx.x >= nb) return;  // uniform
+    uint32_t run = block_base[blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * EMIT_SPAN + threadIdx.x;
+#pragma unroll 1
+    for (int i = 0; i < EMIT_ROUNDS; ++i) {
+        const int64_t k = base + (int64_t)i * EMIT_T;
+        mi_particle p;
+        int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1, cb0 = 0, cb1 = -1;
+        uint32_t cnt = 0;
+        if (k < n) {
+            p = P[k];
+            cnt = cells_of(p, g, tx0, tx1, ty0, ty1, cb0, cb1);
+        }
+        uint32_t round_total;
+        uint32_t at = run + block_excl_scan(cnt, wsum, round_total);
+        run += round_total;
+        if (cnt == 0) continue;
         const uint32_t kg = kernel_group(kind_of(p.r_kind));
         const uint32_t hash = ((uint32_t)k * 2654435761u) >> 24;
         const uint32_t local = (kg << 8) | (mix ? hash : 0u);
@@ -218,7 +307,6 @@ __global__ void k_cell_bounds(const uint32_t* __restrict__ keys, int64_t np, int
 // their exclusive scans, in one work group: thread t takes a contiguous run of
 // cells, the run totals are scanned in LDS, and the offsets written.
 // item_off[ncells] and multi_off[ncells] hold the totals (also status words).
-constexpr int SCAN_T = 1024;
 
 __global__ __launch_bounds__(SCAN_T) void k_item_scan(const uint32_t* __restrict__ start,
                                                       const uint32_t* __restrict__ end,
@@ -329,19 +417,19 @@ hipError_t bin_particles(const mi_particle* d_p, int64_t n, int64_t n_pairs, con
 
     hipLaunchKernelGGL(k_bin_init, dim3(blocks_for(g.ncells > 8 ? g.ncells : 8, T)), dim3(T), 0, s,
                        status, cstart, cend, g.ncells);
-    if (n > 0) {
-        hipLaunchKernelGGL(k_count, dim3(blocks_for(n, T)), dim3(T), 0, s, d_p, n, g, d.weights,
+    const int64_t nb = emit_blocks(n);
+    if (nb > 0)
+        hipLaunchKernelGGL(k_count, dim3((unsigned)nb), dim3(EMIT_T), 0, s, d_p, n, g, d.weights,
                            d.weights_n, counts, status);
-        hipError_t e = rocprim::exclusive_scan(temp, tb, counts, offsets, 0u, (size_t)n,
-                                               rocprim::plus<uint32_t>(), s);
-        if (e != hipSuccess) return e;
-    }
-    int64_t emit_threads = n > n_pairs ? n : n_pairs;
-    if (emit_threads < 1) emit_threads = 1;
-    unsigned eb = blocks_for(emit_threads, T);
-    if (eb > 65536u) eb = 65536u;
-    hipLaunchKernelGGL(k_emit, dim3(eb), dim3(T), 0, s, d_p, n, n_pairs, g, counts, offsets, keys,
-                       vals, status, record_order ? 0 : 1);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_T), 0, s, counts, nb, offsets);
+    // the emit grid: one block per particle block, at least enough threads
+    // (up to 2^24) to fill any padding entries in a few strides
+    int64_t eb = nb;
+    const int64_t pad_blocks = (n_pairs + EMIT_T - 1) / EMIT_T;
+    if (eb < pad_blocks) eb = pad_blocks < 65536 ? pad_blocks : 65536;
+    if (eb < 1) eb = 1;
+    hipLaunchKernelGGL(k_emit, dim3((unsigned)eb), dim3(EMIT_T), 0, s, d_p, n, n_pairs, g, offsets,
+                       keys, vals, status, record_order ? 0 : 1);
     if (n_pairs > 0) {
         // In record order only the cell and the kernel group are sorted (two
         // radix passes up to 2^14 cells); the stable sort keeps the rest.

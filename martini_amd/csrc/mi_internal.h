@@ -40,7 +40,7 @@ __host__ __device__ inline uint32_t cells_of(const mi_particle& p, const Grid& g
 
 // Byte offsets of the workspace regions (each 256-byte aligned).
 struct Layout {
-    size_t counts, offsets;            // uint32[n] each
+    size_t counts, offsets;            // uint32[emit blocks + 2] each: block totals, bases
     size_t keys, vals, keys2, vals2;   // uint32[n_pairs] each; vals2: particle index in cell order
     size_t cell_start, cell_end;       // uint32[ncells] each
     size_t item_off, multi_off;        // uint32[ncells + 1] each: exclusive scans of the
