@@ -36,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"plain": 157.3, "mfma": 2500.0}  # MI355X dense FP32 / BF16 (public figures)
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 PRODUCTS = {"plain": 1, "mfma": 3}  # products per logical multiply-add (bf16x3 spends 3)
 DTYPE = {"plain": "fp32", "mfma": "bf16x3"}
 BOUND = {"plain": "valu_fp32", "mfma": "mfma"}
@@ -206,6 +207,7 @@ def beam_leg(desc, fwhm_px: float, reps: int = 5) -> dict:
             "achieved_tflops": f_sep / (ms_sep * 1e-3) / 1e12,
             "frac_fp32_peak": f_sep / (ms_sep * 1e-3) / 1e12 / PEAK_TFLOPS["plain"],
             "hbm_gbs": 4 * 4.0 * vox / (ms_sep * 1e-3) / 1e9,
+            "frac_hbm_peak": 4 * 4.0 * vox / (ms_sep * 1e-3) / 1e9 / PEAK_HBM_GBS,
             "direct_ms": ms_dir,
             "direct_achieved_tflops": f_dir / (ms_dir * 1e-3) / 1e12}
 

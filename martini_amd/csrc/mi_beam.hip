@@ -13,8 +13,12 @@
 // Direct summation in a fixed order (deterministic): FP32 along each row
 // segment of a block, FP64 across them.  A separable beam (a circular or
 // axis-aligned Gaussian's image is the outer product of two vectors) is
-// convolved by the caller as two such passes, a kx x 1 and a 1 x ky image.
+// convolved by the caller as two such passes, a kx x 1 and a 1 x ky image;
+// those one-dimensional images take k_convolve_line, which reads each pixel's
+// channels contiguously.
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 #include "../../include/martini_mi355x.h"
 #include "mi_internal.h"
@@ -116,6 +120,109 @@ __global__ __launch_bounds__(256) void k_convolve_beam(const float* __restrict__
     }
 }
 
+// One-dimensional pass over a [O][L][E] view of the cube (E contiguous, in
+// units of V: float4 or float), along L:
+//
+//   out[o][l][e] = sum_{a < k} K[a] * in[o][l + k/2 - a][e],  zero outside [0, L).
+//
+// A kx x 1 image is the pass along x (O = 1, L = nx, E = ny * nc), a 1 x ky
+// image the pass along y (O = nx, L = ny, E = nc).  The 64 lanes of a wave
+// take 64 consecutive elements of E (one 1 KB row of float4s per load), and
+// each wave LINE_R consecutive outputs along L: it walks the LINE_R + k - 1
+// input rows once, adding each to the outputs whose taps reach it, so an
+// input row is loaded (LINE_R + k - 1) / LINE_R times per wave instead of k.
+// The taps are held in LDS as FP64 and every sum is formed in FP64 in a fixed
+// order (deterministic; FP32 rounding only at the output).
+constexpr int LINE_R = 8;        // outputs per wave along the axis
+constexpr int LINE_WAVES = 4;    // waves per work group (consecutive strips along L)
+constexpr int LINE_MAX_K = 4096; // taps held in LDS (32 KB); longer images take k_convolve_beam
+constexpr int LINE_G = 4;        // input rows loaded together
+
+template <class V>
+__device__ inline void fma_into(double (&acc)[4], double w, const V& v);
+template <>
+__device__ inline void fma_into<float4>(double (&acc)[4], double w, const float4& v) {
+    acc[0] = fma(w, (double)v.x, acc[0]);
+    acc[1] = fma(w, (double)v.y, acc[1]);
+    acc[2] = fma(w, (double)v.z, acc[2]);
+    acc[3] = fma(w, (double)v.w, acc[3]);
+}
+template <>
+__device__ inline void fma_into<float>(double (&acc)[4], double w, const float& v) {
+    acc[0] = fma(w, (double)v, acc[0]);
+}
+__device__ inline float4 round_out(const double (&a)[4], float4*) {
+    return make_float4((float)a[0], (float)a[1], (float)a[2], (float)a[3]);
+}
+__device__ inline float round_out(const double (&a)[4], float*) { return (float)a[0]; }
+
+template <class V>
+__global__ __launch_bounds__(64 * LINE_WAVES) void k_convolve_line(
+    const V* __restrict__ in, V* __restrict__ out, int L, int64_t E, const float* __restrict__ K,
+    int k) {
+    extern __shared__ double taps[];
+    for (int a = threadIdx.x; a < k; a += 64 * LINE_WAVES) taps[a] = (double)K[a];
+    __syncthreads();
+    const int64_t e = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int l0 = (blockIdx.y * LINE_WAVES + wave) * LINE_R;
+    if (e >= E || l0 >= L) return;
+    const V* src = in + (int64_t)blockIdx.z * L * E + e;
+    V* dst = out + (int64_t)blockIdx.z * L * E + e;
+    double acc[LINE_R][4];
+#pragma unroll
+    for (int q = 0; q < LINE_R; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[q][c] = 0.0;
+    // Input row m0 + r feeds output l0 + q through tap a = q + k - 1 - r; rows
+    // outside [0, L) are zero and skipped.  Rows are taken LINE_G at a time, their
+    // loads issued together.
+    const int m0 = l0 + k / 2 - (k - 1);
+    const int r0 = max(0, -m0), r1 = min(LINE_R + k - 1, L - m0);
+    int r = r0;
+    for (; r + LINE_G <= r1; r += LINE_G) {
+        V v[LINE_G];
+#pragma unroll
+        for (int g = 0; g < LINE_G; ++g) v[g] = src[(int64_t)(m0 + r + g) * E];
+#pragma unroll
+        for (int g = 0; g < LINE_G; ++g)
+#pragma unroll
+            for (int q = 0; q < LINE_R; ++q) {
+                const int a = q + k - 1 - (r + g);
+                if (a >= 0 && a < k) fma_into<V>(acc[q], taps[a], v[g]);  // uniform
+            }
+    }
+    for (; r < r1; ++r) {
+        const V v = src[(int64_t)(m0 + r) * E];
+#pragma unroll
+        for (int q = 0; q < LINE_R; ++q) {
+            const int a = q + k - 1 - r;
+            if (a >= 0 && a < k) fma_into<V>(acc[q], taps[a], v);  // uniform
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LINE_R; ++q)
+        if (l0 + q < L) dst[(int64_t)(l0 + q) * E] = round_out(acc[q], (V*)nullptr);
+}
+
+static hipError_t convolve_line(const float* in, float* out, int64_t O, int L, int64_t E_floats,
+                                const float* K, int k, hipStream_t s) {
+    const unsigned strips = (unsigned)((L + LINE_R * LINE_WAVES - 1) / (LINE_R * LINE_WAVES));
+    const size_t lds = sizeof(double) * (size_t)k;
+    const bool vec = E_floats % 4 == 0 &&
+                     ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    if (vec) {
+        const int64_t E = E_floats / 4;
+        hipLaunchKernelGGL(k_convolve_line<float4>, dim3((unsigned)((E + 63) / 64), strips, (unsigned)O),
+                           dim3(64 * LINE_WAVES), lds, s, reinterpret_cast<const float4*>(in),
+                           reinterpret_cast<float4*>(out), L, E, K, k);
+    } else {
+        hipLaunchKernelGGL(k_convolve_line<float>, dim3((unsigned)((E_floats + 63) / 64), strips, (unsigned)O),
+                           dim3(64 * LINE_WAVES), lds, s, in, out, L, E_floats, K, k);
+    }
+    return hipGetLastError();
+}
+
 size_t beam_lds_bytes(int kx, int ky) {
     const int h = beam_block_rows(kx, ky), w = beam_block_cols(ky);
     return sizeof(float4) * (size_t)(BT - 1 + h) * (BT - 1 + w) + sizeof(float) * (size_t)h * w;
@@ -123,6 +230,10 @@ size_t beam_lds_bytes(int kx, int ky) {
 
 hipError_t convolve_beam(const float* in, float* out, int nx, int ny, int nc, const float* K,
                          int kx, int ky, hipStream_t s) {
+    if (ky == 1 && kx <= LINE_MAX_K)  // along x: [1][nx][ny * nc]
+        return convolve_line(in, out, 1, nx, (int64_t)ny * nc, K, kx, s);
+    if (kx == 1 && ky <= LINE_MAX_K)  // along y: [nx][ny][nc]
+        return convolve_line(in, out, nx, ny, nc, K, ky, s);
     const int h = beam_block_rows(kx, ky), w = beam_block_cols(ky);
     const size_t lds = beam_lds_bytes(kx, ky);
     const unsigned tiles = (unsigned)(((nx + BT - 1) / BT) * ((ny + BT - 1) / BT));
