@@ -61,10 +61,12 @@ __host__ __device__ inline int range_of(int32_t r_kind) { return r_kind >> 8; }
 // The tabulated kernels (MI_KERNEL_CUBIC .. MI_KERNEL_QUARTIC) weigh a pixel
 // zero when q = |pixel - particle| / h >= 1, so their footprint is further cut
 // to the rows and columns within h (1 + 2^-12) of the particle: every pixel
-// left out has q^2 >= 1 + 2^-11, which the weight's FP32 evaluation of q^2
+// left out has q^2 > 1 + 2^-12, which the weight's FP32 evaluation of q^2
 // (a few ulps) cannot bring below 1.  This drops none of the weights the
 // square mask gives and about a fifth of the pixels on typical discs.  The
-// bounds are formed in FP64 without contraction, so host and device agree.
+// bounds are offsets from ix (iy) formed in FP32 from the fraction fx (fy),
+// whose rounding (below 2^-23 of h) the margin covers, without contraction,
+// so host and device agree.
 __host__ __device__ inline bool footprint(const mi_particle& p, int nx, int ny, int& i_lo,
                                           int& i_hi, int& j_lo, int& j_hi) {
 #pragma clang fp contract(off)
@@ -75,10 +77,9 @@ __host__ __device__ inline bool footprint(const mi_particle& p, int nx, int ny, 
     j_lo = p.iy - r + (p.fy > 0.0f ? 1 : 0);
     j_hi = p.iy + r;
     if (kind_of(p.r_kind) <= MI_KERNEL_QUARTIC && p.h > 0.0f && p.h <= (float)r) {
-        const double R = (double)p.h * (1.0 + 0x1p-12);
-        const double x = (double)p.ix + (double)p.fx, y = (double)p.iy + (double)p.fy;
-        const int a = (int)ceil(x - R), b = (int)floor(x + R);
-        const int c = (int)ceil(y - R), d = (int)floor(y + R);
+        const float R = p.h * (1.0f + 0x1p-12f);
+        const int a = p.ix + (int)ceilf(p.fx - R), b = p.ix + (int)floorf(p.fx + R);
+        const int c = p.iy + (int)ceilf(p.fy - R), d = p.iy + (int)floorf(p.fy + R);
         if (a > i_lo) i_lo = a;
         if (b < i_hi) i_hi = b;
         if (c > j_lo) j_lo = c;
@@ -274,18 +275,29 @@ struct WFactors {
     float b, t, n;
 };
 
+// 1 / x in FP64: the hardware reciprocal refined by two Newton steps (within
+// an ulp of the quotient for finite nonzero x); for x = 0 or infinite the
+// hardware value (an infinity or a zero) is the quotient itself.
+__device__ inline double recip(double x) {
+    const double r0 = __builtin_amdgcn_rcp(x);
+    double r = fma(r0, fma(-x, r0, 1.0), r0);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return (x == 0.0 || __builtin_isinf(x)) ? r0 : r;
+}
+
 __device__ inline WFactors weight_factors(int kind, float h, float gt, float ginv) {
     WFactors f{0.0, 0.0, 0.0f, gt, ginv};
     if (kind <= MI_KERNEL_QUARTIC) {
         const double hd = (double)h;
-        f.a = 1.0 / (hd * hd);
+        f.a = recip(hd * hd);
         f.b = (float)f.a;
     } else if (kind == MI_KERNEL_GAUSS) {
         const double hs = (double)h * GAUSS_SIG;
-        f.a = 1.0 / (hs * hs);
+        const double rh = recip(hs);
+        f.a = rh * rh;
         const double edge = (double)gt * hs + 0.70710678118654752;
         f.e = edge * edge;
-        f.b = (float)(1.0 / (hs * 1.41421356237309505));
+        f.b = (float)(rh * 0.70710678118654752);  // 1 / (hs sqrt 2)
     }
     return f;
 }

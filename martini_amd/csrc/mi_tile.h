@@ -53,7 +53,6 @@ struct ChunkState {
     double ge[KC];      // Gaussian kernel: WFactors::e
     float4 gf[KC];      // Gaussian kernel: WFactors::b, t, n
     int ilo[KC], ihi[KC];
-    int wdiv[KC];       // ceil(2^16 / footprint width): (n * wdiv) >> 16 == n / width for n < 4096
     int off[KC + 1];    // exclusive scan of the per-particle pair counts
     int toff[KC + 1];   // exclusive scan of the threads given to each particle's pairs
     int per;            // pairs per thread
@@ -146,28 +145,23 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
                                    const Grid& g, const mi_cube_desc& d, ChunkState& cs) {
     const int lane = threadIdx.x & 63;
     const mi_particle p = unpack_record(r);
-    int cnt = 0, ilo = 1, ihi = 0, jlo = 1, jhi = 0;
-    uint32_t rows = 0, chans = 0;
-    TableWalk tw{0, 0};
-    if (valid && footprint(p, g.nx, g.ny, ilo, ihi, jlo, jhi)) {
-        // a table particle's weights are laid out over its cube-clipped footprint
-        tw.stride = jhi - jlo + 1;
-        tw.base = (int64_t)table_offset(p) - (int64_t)ilo * tw.stride - jlo;
-        ilo = max(ilo, x0);
-        ihi = min(ihi, x0 + TILE - 1);
-        jlo = max(jlo, y0);
-        jhi = min(jhi, y0 + TILE - 1);
-        if (ilo <= ihi && jlo <= jhi) {
-            cnt = (ihi - ilo + 1) * (jhi - jlo + 1);
-            rows = (2u << (ihi - x0)) - (1u << (ilo - x0));
-            const int a = max(p.c_lo, c0) - c0, b = min(p.c_hi, c0 + CBLOCK - 1) - c0;
-            if (a <= b) chans = (2u << (b >> 4)) - (1u << (a >> 4));
-        } else {
-            ilo = 1; ihi = 0; jlo = 1; jhi = 0;
-        }
-    } else {
-        ilo = 1; ihi = 0; jlo = 1; jhi = 0;
-    }
+    // Straight-line code with selects: every lane forms the clipped footprint,
+    // lanes without a particle or a footprint in the tile end with cnt = 0.
+    int fi_lo = 0, fi_hi = -1, fj_lo = 0, fj_hi = -1;
+    const bool fp = footprint(p, g.nx, g.ny, fi_lo, fi_hi, fj_lo, fj_hi);
+    // a table particle's weights are laid out over its cube-clipped footprint
+    TableWalk tw;
+    tw.stride = fj_hi - fj_lo + 1;
+    tw.base = (int64_t)table_offset(p) - (int64_t)fi_lo * tw.stride - fj_lo;
+    const int ci_lo = max(fi_lo, x0), ci_hi = min(fi_hi, x0 + TILE - 1);
+    const int cj_lo = max(fj_lo, y0), cj_hi = min(fj_hi, y0 + TILE - 1);
+    const bool ok = valid && fp && ci_lo <= ci_hi && cj_lo <= cj_hi;
+    const int ilo = ok ? ci_lo : 1, ihi = ok ? ci_hi : 0;
+    const int jlo = ok ? cj_lo : 1, jhi = ok ? cj_hi : 0;
+    const int cnt = ok ? (ihi - ilo + 1) * (jhi - jlo + 1) : 0;
+    uint32_t rows = ok ? (2u << (ihi - x0)) - (1u << (ilo - x0)) : 0u;
+    const int ca = max(p.c_lo, c0) - c0, cb = min(p.c_hi, c0 + CBLOCK - 1) - c0;
+    uint32_t chans = ok && ca <= cb ? (2u << (cb >> 4)) - (1u << (ca >> 4)) : 0u;
     const int incl = wave_scan_add(cnt);
     rows = wave_or(rows);
     chans = wave_or(chans);
@@ -177,13 +171,15 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
     // ceil(total / (NT - KC)): at most total / per + KC <= NT threads in all.
     const int total = __builtin_amdgcn_readlane(incl, 63);
     const int per = max(1, (total + (NT - KC) - 1) / (NT - KC));
-    const int tincl = wave_scan_add((cnt + per - 1) / per);
+    // ceil(cnt / per) = floor((cnt + per - 1 + 1/2) / per): the quotient lies at
+    // least 1/74 from an integer (per <= 37), far beyond the reciprocal's error
+    const float rper = __builtin_amdgcn_rcpf((float)per);
+    const int tincl = wave_scan_add((int)(((float)(cnt + per - 1) + 0.5f) * rper));
     if (lane < KC) {
         if (valid) cs.p[lane] = p;
         cs.ilo[lane] = ilo;
         cs.ihi[lane] = ihi;
         const int width = max(jhi - jlo + 1, 1);
-        cs.wdiv[lane] = (65536 + width - 1) / width;
         cs.off[lane + 1] = incl;
         cs.toff[lane + 1] = tincl;
         if (cnt > 0) {
@@ -291,7 +287,9 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
     const int width = w.kw >> 8, kind = w.kw & 255;
     const int l0 = (t - cs.toff[k]) * cs.per;
     const int l1 = active ? min(l0 + cs.per, cs.off[k + 1] - cs.off[k]) : l0;
-    int a = (l0 * cs.wdiv[k]) >> 16;
+    // l0 / width: (l0 + 1/2) / width lies at least 1/32 from an integer (l0 <
+    // 256, width <= 16), far beyond the reciprocal's error
+    int a = (int)(((float)l0 + 0.5f) * __builtin_amdgcn_rcpf((float)width));
     int b = l0 - a * width;
     const int ti = (w.ij & 0xffff) - x0, tj = (w.ij >> 16) - y0;  // footprint corner in the tile
     const double px = w.px - (double)x0, py = w.py - (double)y0;   // particle in tile coordinates

@@ -179,21 +179,22 @@ def test_dirac_spectrum_window_edge_inclusive():
 def _footprints(rec, nx, ny):
     """The device's clipped footprint (mi_device.h footprint) in numpy: the
     square mask of range r, cut for the tabulated kernels to the rows and
-    columns within h (1 + 2^-12) of the particle."""
+    columns within h (1 + 2^-12) of the particle, as offsets from ix (iy)
+    formed in FP32 from the fraction fx (fy)."""
     rr = rec["r_kind"] >> 8
     ilo = rec["ix"] - rr + (rec["fx"] > 0)
     ihi = rec["ix"] + rr
     jlo = rec["iy"] - rr + (rec["fy"] > 0)
     jhi = rec["iy"] + rr
-    h = rec["h"].astype(float)
-    cut = ((rec["r_kind"] & 15) <= _lib.KERNEL_QUARTIC) & (h > 0) & (h <= rr)
-    R = h * (1.0 + 2.0**-12)
-    x = rec["ix"] + rec["fx"].astype(float)
-    y = rec["iy"] + rec["fy"].astype(float)
-    ilo = np.where(cut, np.maximum(ilo, np.ceil(x - R)), ilo).astype(np.int64)
-    ihi = np.where(cut, np.minimum(ihi, np.floor(x + R)), ihi).astype(np.int64)
-    jlo = np.where(cut, np.maximum(jlo, np.ceil(y - R)), jlo).astype(np.int64)
-    jhi = np.where(cut, np.minimum(jhi, np.floor(y + R)), jhi).astype(np.int64)
+    h = rec["h"]
+    cut = ((rec["r_kind"] & 15) <= _lib.KERNEL_QUARTIC) & (h > 0) & (h <= rr.astype(np.float32))
+    R = h * np.float32(1.0 + 2.0**-12)  # FP32 throughout
+    fx, fy = rec["fx"], rec["fy"]
+    ix, iy = rec["ix"].astype(np.int64), rec["iy"].astype(np.int64)
+    ilo = np.where(cut, np.maximum(ilo, ix + np.ceil(fx - R).astype(np.int64)), ilo)
+    ihi = np.where(cut, np.minimum(ihi, ix + np.floor(fx + R).astype(np.int64)), ihi)
+    jlo = np.where(cut, np.maximum(jlo, iy + np.ceil(fy - R).astype(np.int64)), jlo)
+    jhi = np.where(cut, np.minimum(jhi, iy + np.floor(fy + R).astype(np.int64)), jhi)
     ilo, jlo = np.maximum(ilo, 0), np.maximum(jlo, 0)
     ihi, jhi = np.minimum(ihi, nx - 1), np.minimum(jhi, ny - 1)
     return rr, ilo, ihi, jlo, jhi, cut
