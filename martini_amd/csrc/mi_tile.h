@@ -54,7 +54,9 @@ struct ChunkState {
     float4 gf[KC];      // Gaussian kernel: WFactors::b, t, n
     int ilo[KC], ihi[KC];
     int off[KC + 1];    // exclusive scan of the per-particle pair counts
-    int toff[KC + 1];   // exclusive scan of the threads given to each particle's pairs
+    // exclusive scan of the threads given to each particle's pairs (16-byte
+    // aligned: fill_weights reads it eight entries at a time)
+    __attribute__((aligned(16))) int toff[KC + 1];
     int per;            // pairs per thread
     uint32_t rowmask;   // tile rows (x offsets) touched by any footprint of the chunk
     uint32_t chanmask;  // 16-channel sub-blocks of the cell touched by any window
@@ -277,11 +279,17 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
                                     const float* __restrict__ uw, Store store, Store2 store2) {
     const int t = threadIdx.x;
     const bool active = t < cs.toff[KC];
+    // k = the last particle with toff[k] <= t (toff is non-decreasing, and
+    // particles without pairs share their successor's offset), in two rounds of
+    // LDS reads: the eighth it lies in, then the entries of that eighth.
+    static_assert(KC == 32, "the two-round search covers 32 particles");
     int k = 0;
     if (active) {
-#pragma unroll
-        for (int step = KC / 2; step > 0; step >>= 1)
-            if (cs.toff[k + step] <= t) k += step;
+        const int m = 8 * ((cs.toff[8] <= t) + (cs.toff[16] <= t) + (cs.toff[24] <= t));
+        const int4 u = *reinterpret_cast<const int4*>(cs.toff + m);
+        const int4 v = *reinterpret_cast<const int4*>(cs.toff + m + 4);
+        k = m + (u.y <= t) + (u.z <= t) + (u.w <= t) + (v.x <= t) + (v.y <= t) + (v.z <= t) +
+            (v.w <= t);
     }
     const WalkRec w = cs.walk[k];
     const int width = w.kw >> 8, kind = w.kw & 255;
