@@ -300,8 +300,9 @@ int mi_prepare(const double* d_xyz, const double* d_vxyz, const double* d_mHI,
  * and drops the pad after.  Any beam size: the image is taken in blocks that
  * fit in LDS.  A separable image (the outer product of two vectors, as a
  * circular or axis-aligned Gaussian beam's is) convolves faster as two calls,
- * with the kx x 1 and the 1 x ky factor.  Asynchronous.  MI_ERR_LIMIT for
- * dimensions above 32767.
+ * with the kx x 1 and the 1 x ky factor: one-dimensional images of up to 4096
+ * taps take a pass that reads each pixel's channels contiguously and sums in
+ * FP64.  Asynchronous.  MI_ERR_LIMIT for dimensions above 32767.
  */
 int mi_convolve_beam(const float* d_in, float* d_out, int32_t nx, int32_t ny, int32_t nc,
                      const float* d_kernel, int32_t kx, int32_t ky, void* stream);
