@@ -129,6 +129,21 @@ class DataCube:
         z = crpix[2] - 1.0 + (spec - crval[2]) / cdelt[2]
         return np.vstack((x, y, z))
 
+    def pixel_to_world(self, x, y, z):
+        """World coordinates of 0-based pixel positions, the inverse of
+        world_to_pixel (WCS.all_pix2world with origin 0, as write_hdf5 uses it,
+        martini.py:1229-1244): right ascension in [0, 360) and declination in
+        degrees through the inverse gnomonic projection about (ra, dec), and
+        the spectral value in the axis unit (m/s, or Hz in frequency mode)."""
+        crpix, cdelt, crval = self.crpix, self.cdelt, self.crval
+        xi = np.deg2rad((np.asarray(x, float) + 1.0 - crpix[0]) * cdelt[0])
+        eta = np.deg2rad((np.asarray(y, float) + 1.0 - crpix[1]) * cdelt[1])
+        d0 = np.deg2rad(self.dec)
+        den = np.cos(d0) - eta * np.sin(d0)
+        ra = np.rad2deg(np.deg2rad(self.ra) + np.arctan2(xi, den)) % 360.0
+        dec = np.rad2deg(np.arctan2(np.sin(d0) + eta * np.cos(d0), np.hypot(xi, den)))
+        return ra, dec, self._spectral_world(z)
+
     # ---------------------------------------------------------- padding
     def add_pad(self, pad) -> None:
         """Grow the spatial extent by pad pixels per side (datacube.py:669-708)."""

@@ -16,7 +16,8 @@ and have no effect.  A beam pads the cube at construction and
 ``add_noise`` adds a noise model's realisation (martini.py:903-937), drawn
 with numpy's generator as the reference draws it.  ``write_fits`` /
 ``write_beam_fits`` write FITS files (martini.py:939-1152), the cube's data
-unit made on the GPU.  The HDF5 writer is not part of this package.
+unit made on the GPU; ``write_hdf5`` writes the HDF5 file of
+martini.py:1154-1366 through the HDF5 C library (``hdf5.py``).
 """
 
 from __future__ import annotations
@@ -25,8 +26,11 @@ import warnings
 
 import numpy as np
 
-from . import _lib, _pack, fits
+from . import __version__, _lib, _pack, fits, hdf5
 from .insertion import default_inserter
+
+# str(unit) of the cube array, as astropy prints the units the reference uses
+_UNIT_STR = {"Jy/beam": "Jy / beam", "Jy/arcsec2": "Jy / arcsec2", "Jy/pix2": "Jy / pix2"}
 
 
 class _BaseMartini:
@@ -294,6 +298,85 @@ class Martini(_BaseMartini):
         # the reference writes kernel[..., newaxis].T: FITS axis 1 runs along kernel axis 0
         data = np.ascontiguousarray(k[..., np.newaxis].T).astype(">f8").tobytes()
         fits.write_hdu(filename, fits.beam_cards(self._datacube, self.beam, -64), data, True)
+
+    def write_hdf5(self, filename: str, overwrite: bool = True, memmap: bool = False,
+                   compact: bool = False, channels=None):
+        """Write the cube and beam to an HDF5 file (martini.py:1154-1366): the
+        pad is dropped and '.hdf5' appended unless present; the datasets and
+        attributes are the reference's, in its order.  Without ``compact`` the
+        world coordinates of every voxel centre (RA, Dec, channel_mids) and
+        vertex (*_vertices) and the four channel arrays are written too.  With
+        ``memmap`` the file is built in memory and returned open
+        (``hdf5.File``); otherwise it is closed and None returned."""
+        if channels is not None:
+            warnings.warn(DeprecationWarning(
+                "`channels` argument to `write_fits` ignored, channels and their units now "
+                "fixed at DataCube initialization."))
+        dc = self._datacube
+        dc.drop_pad()
+        filename = filename if filename[-5:] == ".hdf5" else filename + ".hdf5"
+        ctype3, cunit3 = fits._spectral_axis(dc)
+        crpix, cdelt, crval = dc.crpix, dc.cdelt, dc.crval
+        f = hdf5.create(filename, overwrite, memmap)
+        try:
+            f.create_dataset("FluxCube", np.asarray(dc._array, np.float64).squeeze())
+            if not compact:
+                nx, ny, nc = dc.n_px_x, dc.n_px_y, dc.n_channels
+
+                def grids(offset, n):
+                    # the sky axes do not depend on the channel, nor the spectral on the pixel:
+                    # broadcast, then materialised one dataset at a time
+                    x = (np.arange(n[0]) + offset)[:, None, None]
+                    y = (np.arange(n[1]) + offset)[None, :, None]
+                    z = (np.arange(n[2]) + offset)[None, None, :]
+                    return (np.broadcast_to(g, n) for g in dc.pixel_to_world(x, y, z))
+
+                for names, n, off in ((("RA", "Dec", "channel_mids"), (nx, ny, nc), 0.0),
+                                      (("RA_vertices", "Dec_vertices", "channel_vertices"),
+                                       (nx + 1, ny + 1, nc + 1), -0.5)):
+                    for name, grid, unit in zip(names, grids(off, n), ("deg", "deg", cunit3)):
+                        f.create_dataset(name, grid, {"Unit": unit})
+                for name, unit in (("velocity_channel_mids", "m / s"),
+                                   ("velocity_channel_edges", "m / s"),
+                                   ("frequency_channel_mids", "Hz"),
+                                   ("frequency_channel_edges", "Hz")):
+                    f.create_dataset(name, getattr(dc, name), {"Unit": unit})
+            axes = {"RA": (1, "RA---TAN", "deg"), "Dec": (2, "DEC--TAN", "deg"),
+                    "V": (3, ctype3, cunit3)}
+            cube_attrs = {"AxisOrder": "(RA,Dec,Channels)", "FluxCubeUnit": _UNIT_STR[dc._array_unit]}
+            for ax, (i, ctype, cunit) in axes.items():
+                cube_attrs.update({f"delta{ax}_in_{ax}Unit": float(cdelt[i - 1]),
+                                   f"{ax}0_in_px": float(crpix[i - 1]) - 1.0,
+                                   f"{ax}0_in_{ax}Unit": float(crval[i - 1]),
+                                   f"{ax}Unit": cunit, f"{ax}ProjType": ctype})
+            cube_attrs["SpecSys"] = dc.specsys.upper()
+            beam = self.beam
+            beam_deg = {}
+            if beam is not None:
+                beam_deg = {"BeamPA": float(beam.bpa), "BeamMajor_in_deg": float(beam.bmaj) / 3600.0,
+                            "BeamMinor_in_deg": float(beam.bmin) / 3600.0}
+            stamp = {"DateCreated": hdf5.now_string(), "MartiniVersion": f"martini_amd {__version__}",
+                     "AstropyVersion": "not used"}
+            f.set_attrs("FluxCube", {**cube_attrs, **beam_deg, **stamp})
+            if beam is not None:
+                if beam.kernel is None:
+                    raise ValueError("Martini.write_hdf5: Called with beam present but beam kernel"
+                                     " uninitialized.")
+                k = np.asarray(beam.kernel, np.float64)
+                battrs = {"BeamUnit": ""}  # the beam image is dimensionless (beams.py:145-157)
+                for ax, (i, ctype, cunit) in axes.items():
+                    battrs.update({f"delta{ax}_in_{ax}Unit": float(cdelt[i - 1]),
+                                   f"{ax}0_in_px": k.shape[i - 1] // 2 if i < 3 else 0,
+                                   f"{ax}0_in_{ax}Unit": float(crval[i - 1]),
+                                   f"{ax}Unit": cunit, f"{ax}ProjType": ctype})
+                f.create_dataset("Beam", k[..., np.newaxis], {**battrs, **beam_deg, **stamp})
+        except BaseException:
+            f.close()
+            raise
+        if memmap:
+            return f
+        f.close()
+        return None
 
     def insert_source_in_cube(self, skip_validation: bool = False, progressbar=None,
                               ncpu: int = 1, method: str = "plain") -> None:
