@@ -4,9 +4,10 @@
 // (martini/martini.py:271-274): every particle is listed once in each cell
 // its footprint and channel window touch, and the list is sorted by
 // (cell, kernel group, local order) with a stable radix sort, so that each
-// cell's particles form one contiguous, deterministic segment.  The local order
-// is the pixel's Morton code for the MFMA form (spatially compact chunks) and a
-// hash of the particle index for the FP32 form (chunks spread over the tile).
+// cell's particles form one contiguous, deterministic segment.  The MFMA form
+// keeps the record order within a cell (its records are laid out by tile and
+// pixel Z order, so its chunks are spatially compact); the FP32 form orders
+// by a hash of the particle index (chunks spread over the tile).
 // Long segments are cut into work items of at most item_length() particles;
 // such cells get one partial tile per item.
 #include <hip/hip_runtime.h>
