@@ -269,7 +269,7 @@ __device__ inline float gauss_weight(double dx, double dy, double fa, double fe,
 // store2(k, pixel1, w1, pixel2, w2).  Particle k's pairs (pixel row l / width,
 // column l % width of its clipped footprint, l < cnt) are split over threads
 // toff[k] .. toff[k+1] - 1, `per` consecutive pairs each, so a thread stays on
-// one particle: one binary search, then a walk over its pixels.  When every
+// one particle: one search of the thread scan, then a walk over its pixels.  When every
 // particle of the wave's pairs uses the item's LDS table, the walk is
 // branch-free and takes two pixels per step, whose evaluations overlap.
 template <class Store, class Store2>
