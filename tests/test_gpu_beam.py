@@ -126,3 +126,64 @@ def test_circular_beam_pipeline_matches_oracle():
     voxel, flux = parity(out[True], ref)
     print(f"circular beam pipeline: voxel {voxel:.3e} flux {flux:.3e} (image {beam.kernel.shape})")
     assert voxel <= VOXEL_TOL and flux <= FLUX_TOL
+
+
+@pytest.mark.parametrize("shape,k,axis", [
+    ((33, 29, 7), 9, 0),      # channels not a multiple of 4: the FP32-lane pass
+    ((33, 29, 7), 9, 1),
+    ((20, 24, 12), 1, 0),     # a one-tap image: a scaled copy
+    ((40, 36, 8), 61, 1),     # an image longer than the map: zero boundary on both ends
+    ((9, 4100, 4), 4101, 1),  # longer than the one-dimensional pass holds: the 2D kernel
+])
+def test_one_dimensional_images(shape, k, axis):
+    """kx x 1 and 1 x ky images (the separable passes) against a direct FP64
+    sum, including the cases the one-dimensional kernel hands to the 2D one."""
+    import torch
+
+    rng = np.random.default_rng(k + axis)
+    cube = rng.random(shape)
+    taps = rng.random(k)
+    taps /= taps.sum()
+    d_in = torch.from_numpy(cube.astype(np.float32)).cuda()
+    d_out = torch.empty_like(d_in)
+    d_k = torch.from_numpy(taps.astype(np.float32)).cuda()
+    kx, ky = (k, 1) if axis == 0 else (1, k)
+    _lib.convolve_beam(d_in, d_out, *shape, d_k, kx, ky)
+    gpu = d_out.cpu().numpy().astype(float)
+    c32 = cube.astype(np.float32).astype(float)
+    t32 = taps.astype(np.float32).astype(float)
+    n = shape[axis]
+    ref = np.zeros(shape)
+    for a in range(k):  # out[l] += K[a] in[l + k/2 - a], zero outside
+        s = k // 2 - a
+        lo, hi = max(0, -s), min(n, n - s)
+        if lo >= hi:
+            continue
+        src = [slice(None)] * 3
+        dst = [slice(None)] * 3
+        dst[axis] = slice(lo, hi)
+        src[axis] = slice(lo + s, hi + s)
+        ref[tuple(dst)] += t32[a] * c32[tuple(src)]
+    err = np.abs(gpu - ref).max() / np.abs(ref).max()
+    print(f"{shape} k={k} axis={axis}: max err {err:.2e}")
+    assert err <= 1e-6
+
+
+def test_one_dimensional_pass_unaligned_buffers():
+    """Cube pointers off 16-byte alignment take the FP32-lane pass and give
+    the same result as the aligned vector pass."""
+    import torch
+
+    rng = np.random.default_rng(3)
+    shape = (24, 20, 16)
+    cube = rng.random(shape).astype(np.float32)
+    taps = torch.from_numpy(rng.random(7).astype(np.float32)).cuda()
+    base_in = torch.zeros(cube.size + 1, dtype=torch.float32, device="cuda")
+    base_out = torch.zeros(cube.size + 1, dtype=torch.float32, device="cuda")
+    base_in[1:] = torch.from_numpy(cube.ravel()).cuda()
+    d_in, d_out = base_in[1:], base_out[1:]  # 4-byte offset
+    _lib.convolve_beam(d_in, d_out, *shape, taps, 1, 7)
+    aligned_in = torch.from_numpy(cube).cuda()
+    aligned_out = torch.empty_like(aligned_in)
+    _lib.convolve_beam(aligned_in, aligned_out, *shape, taps, 1, 7)
+    assert torch.equal(d_out.reshape(shape), aligned_out)
