@@ -308,24 +308,37 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
         // from the FP64 form, far below the table's own error.
         const float gx = (float)(px - (double)ti), gy = (float)(py - (double)tj);
         const float fa = (float)w.fa;
+        // Pixel l of the footprint lies in row floor((l + 1/2) / width) (exact, as
+        // for a above) and column l - row * width, all small integers formed in
+        // FP32: no wrap tests, and the offsets from the particle need no
+        // conversion.  Its index in the tile is row * TILE + column + corner.
+        const float rw = __builtin_amdgcn_rcpf((float)width), wf = (float)width;
+        const float corner = (float)(ti * TILE + tj);
+        auto at = [&](float lf, float& af, float& bf) {
+            af = floorf((lf + 0.5f) * rw);
+            bf = fmaf(-af, wf, lf);
+        };
+        auto px_of = [&](float af, float bf) { return (int)fmaf(af, (float)TILE, bf + corner); };
         auto walk = [&](auto cubic) {
-            auto eval = [&](int ra, int rb) {
-                const float dx = gx - (float)ra, dy = gy - (float)rb;
+            auto eval = [&](float af, float bf) {
+                const float dx = gx - af, dy = gy - bf;
                 return table_weight_select<decltype(cubic)::value>(tab, fmaf(dx, dx, dy * dy) * fa,
                                                                    fa);
             };
-            int l = l0;
-            for (; l + 1 < l1; l += 2) {
-                const int a1 = a, b1 = b;
-                b = b + 1 == width ? 0 : b + 1;
-                a = b == 0 ? a + 1 : a;
-                const int a2 = a, b2 = b;
-                b = b + 1 == width ? 0 : b + 1;
-                a = b == 0 ? a + 1 : a;
+            float lf = (float)l0;
+            const float lend = (float)l1;
+            for (; lf + 1.0f < lend; lf += 2.0f) {
+                float a1, b1, a2, b2;
+                at(lf, a1, b1);
+                at(lf + 1.0f, a2, b2);
                 const float v1 = eval(a1, b1), v2 = eval(a2, b2);
-                store2(k, (ti + a1) * TILE + tj + b1, v1, (ti + a2) * TILE + tj + b2, v2);
+                store2(k, px_of(a1, b1), v1, px_of(a2, b2), v2);
             }
-            if (l < l1) store(k, (ti + a) * TILE + tj + b, eval(a, b));
+            if (lf < lend) {
+                float a1, b1;
+                at(lf, a1, b1);
+                store(k, px_of(a1, b1), eval(a1, b1));
+            }
         };
         if (tab_kind == MI_KERNEL_CUBIC)  // uniform
             walk(std::true_type{});
