@@ -313,12 +313,16 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
         // FP32: no wrap tests, and the offsets from the particle need no
         // conversion.  Its index in the tile is row * TILE + column + corner.
         const float rw = __builtin_amdgcn_rcpf((float)width), wf = (float)width;
-        const float corner = (float)(ti * TILE + tj);
+        const float hrw = 0.5f * rw, corner4 = (float)(4 * (ti * TILE + tj));
         auto at = [&](float lf, float& af, float& bf) {
-            af = floorf((lf + 0.5f) * rw);
+            af = floorf(fmaf(lf, rw, hrw));  // (l + 1/2) / width, one rounding
             bf = fmaf(-af, wf, lf);
         };
-        auto px_of = [&](float af, float bf) { return (int)fmaf(af, (float)TILE, bf + corner); };
+        auto px_of = [&](float af, float bf) {
+            const int q = (int)fmaf(af, (float)(4 * TILE), fmaf(bf, 4.0f, corner4));  // 4 * index
+            __builtin_assume((q & 3) == 0);
+            return q >> 2;
+        };
         auto walk = [&](auto cubic) {
             auto eval = [&](float af, float bf) {
                 const float dx = gx - af, dy = gy - bf;
