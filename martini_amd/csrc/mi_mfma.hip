@@ -160,23 +160,36 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                      chans);
         __syncthreads();
         if (rows != 0u) {
+            // The wave's four tile rows held together: each S operand read serves all.
+            bf16x8 ah[4], al[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                ah[r] = bf16x8{};
+                al[r] = bf16x8{};
+                if (!(rows & (1u << r))) continue;
+                const int row = (wave * 4 + r) * TILE + m;
+                ah[r] = get8(Ah, row, kg);
+                al[r] = get8(Al, row, kg);
+            }
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                if (!(chans & (1u << nb))) continue;
+                const bf16x8 bh = get8(Bh, nb * 16 + m, kg), bl = get8(Bl, nb * 16 + m, kg);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if (!(rows & (1u << r))) continue;
+                    floatx4 c = acc[r][nb];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh, c, 0, 0, 0);
+                    acc[r][nb] = c;
+                }
+            }
+            // Clear the rows' 1 KiB in both W tiles (one 16-byte store per lane
+            // each); the reads above come first in this wave's LDS order.
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 if (!(rows & (1u << r))) continue;
-                const int row = (wave * 4 + r) * TILE + m;
-                const bf16x8 ah = get8(Ah, row, kg), al = get8(Al, row, kg);
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb) {
-                    if (!(chans & (1u << nb))) continue;
-                    const bf16x8 bh = get8(Bh, nb * 16 + m, kg), bl = get8(Bl, nb * 16 + m, kg);
-                    floatx4 c = acc[r][nb];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
-                    acc[r][nb] = c;
-                }
-                // Clear the row's 1 KiB in both W tiles (one 16-byte store per lane each);
-                // the reads above come first in this wave's LDS order.
                 const int zb = (wave * 4 + r) * TILE * KROW + lane * 16;
                 *reinterpret_cast<uint4*>(Ah + zb) = make_uint4(0u, 0u, 0u, 0u);
                 *reinterpret_cast<uint4*>(Al + zb) = make_uint4(0u, 0u, 0u, 0u);

@@ -47,11 +47,18 @@ __device__ inline TableWalk table_walk(const WalkRec& w) {
     return TableWalk{__double_as_longlong(w.px), __double_as_longlong(w.py)};
 }
 
+// What the Gaussian walk needs beyond the WalkRec: WFactors::b, t, n and e,
+// at the WalkRec's 32-byte stride.
+struct __attribute__((aligned(16))) GaussRec {
+    float4 f;  // b, t, n, 0
+    double e;
+    double unused;
+};
+
 struct ChunkState {
     mi_particle p[KC];
     WalkRec walk[KC];
-    double ge[KC];      // Gaussian kernel: WFactors::e
-    float4 gf[KC];      // Gaussian kernel: WFactors::b, t, n
+    GaussRec gr[KC];    // Gaussian kernel factors
     int ilo[KC], ihi[KC];
     int off[KC + 1];    // exclusive scan of the per-particle pair counts
     // exclusive scan of the threads given to each particle's pairs (16-byte
@@ -199,8 +206,7 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
             w.kw = kind | (width << 8);
             cs.walk[lane] = w;
             if (kind == MI_KERNEL_GAUSS) {
-                cs.ge[lane] = f.e;
-                cs.gf[lane] = make_float4(f.b, f.t, f.n, 0.0f);
+                cs.gr[lane] = GaussRec{make_float4(f.b, f.t, f.n, 0.0f), f.e, 0.0};
             }
         }
     }
@@ -243,7 +249,7 @@ __device__ inline void fill_gauss_tables(const ChunkState& cs, GaussTables& gt) 
     const int k = threadIdx.x & (KC - 1), slot = threadIdx.x / KC;
     const bool gk = (cs.gmask >> k) & 1u;
     const WalkRec w = cs.walk[k];
-    const float bf = cs.gf[k].x;
+    const float bf = cs.gr[k].f.x;
     const int i0 = w.ij & 0xffff, j0 = w.ij >> 16;
     const int rows = gk ? cs.ihi[k] - i0 + 1 : 0, cols = gk ? w.kw >> 8 : 0;
     for (int r = slot; __any(r < rows); r += NT / KC)
@@ -366,8 +372,8 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
     }
     if (kind == MI_KERNEL_GAUSS || !active) {
         // Gaussian particles: the separable factors come from the chunk's tables
-        const double fe = cs.ge[k];
-        const float4 gg = cs.gf[k];
+        const double fe = cs.gr[k].e;
+        const float4 gg = cs.gr[k].f;
         for (int l = l0; l < l1; ++l) {
             const int i = x0 + ti + a, j = y0 + tj + b;
             store(k, (ti + a) * TILE + tj + b,
