@@ -288,9 +288,9 @@ __device__ inline double recip(double x) {
 __device__ inline WFactors weight_factors(int kind, float h, float gt, float ginv) {
     WFactors f{0.0, 0.0, 0.0f, gt, ginv};
     if (kind <= MI_KERNEL_QUARTIC) {
-        const double hd = (double)h;
-        f.a = recip(hd * hd);
-        f.b = (float)f.a;
+        // 1/h^2 within two FP32 ulps (its uses round q^2 to FP32)
+        f.b = __builtin_amdgcn_rcpf(h * h);
+        f.a = (double)f.b;
     } else if (kind == MI_KERNEL_GAUSS) {
         const double hs = (double)h * GAUSS_SIG;
         const double rh = recip(hs);

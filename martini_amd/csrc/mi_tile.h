@@ -158,22 +158,18 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
     // lanes without a particle or a footprint in the tile end with cnt = 0.
     int fi_lo = 0, fi_hi = -1, fj_lo = 0, fj_hi = -1;
     const bool fp = footprint(p, g.nx, g.ny, fi_lo, fi_hi, fj_lo, fj_hi);
-    // a table particle's weights are laid out over its cube-clipped footprint
-    TableWalk tw;
-    tw.stride = fj_hi - fj_lo + 1;
-    tw.base = (int64_t)table_offset(p) - (int64_t)fi_lo * tw.stride - fj_lo;
     const int ci_lo = max(fi_lo, x0), ci_hi = min(fi_hi, x0 + TILE - 1);
     const int cj_lo = max(fj_lo, y0), cj_hi = min(fj_hi, y0 + TILE - 1);
     const bool ok = valid && fp && ci_lo <= ci_hi && cj_lo <= cj_hi;
     const int ilo = ok ? ci_lo : 1, ihi = ok ? ci_hi : 0;
     const int jlo = ok ? cj_lo : 1, jhi = ok ? cj_hi : 0;
     const int cnt = ok ? (ihi - ilo + 1) * (jhi - jlo + 1) : 0;
-    uint32_t rows = ok ? (2u << (ihi - x0)) - (1u << (ilo - x0)) : 0u;
+    const uint32_t rows = ok ? (2u << (ihi - x0)) - (1u << (ilo - x0)) : 0u;
     const int ca = max(p.c_lo, c0) - c0, cb = min(p.c_hi, c0 + CBLOCK - 1) - c0;
-    uint32_t chans = ok && ca <= cb ? (2u << (cb >> 4)) - (1u << (ca >> 4)) : 0u;
+    const uint32_t chans = ok && ca <= cb ? (2u << (cb >> 4)) - (1u << (ca >> 4)) : 0u;
     const int incl = wave_scan_add(cnt);
-    rows = wave_or(rows);
-    chans = wave_or(chans);
+    static_assert(TILE <= 16 && CBLOCK / 16 <= 16, "rows and channel blocks share one word");
+    const uint32_t masks = wave_or(rows | (chans << 16));  // one reduction for both
     const uint32_t gauss = (uint32_t)__ballot(lane < KC && cnt > 0 &&
                                               kind_of(p.r_kind) == MI_KERNEL_GAUSS);
     // Each particle's pairs go to ceil(cnt / per) threads of their own, per =
@@ -198,8 +194,10 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
             w.px = (double)p.ix + (double)p.fx;
             w.py = (double)p.iy + (double)p.fy;
             if (kind == MI_KERNEL_TABLE) {
-                w.px = __longlong_as_double(tw.base);
-                w.py = __longlong_as_double(tw.stride);
+                // its weights are laid out over its cube-clipped footprint
+                const int64_t stride = fj_hi - fj_lo + 1;
+                w.px = __longlong_as_double((int64_t)table_offset(p) - (int64_t)fi_lo * stride - fj_lo);
+                w.py = __longlong_as_double(stride);
             }
             w.fa = f.a;
             w.ij = ilo | (jlo << 16);
@@ -214,8 +212,8 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
         cs.off[0] = 0;
         cs.toff[0] = 0;
         cs.per = per;
-        cs.rowmask = rows;
-        cs.chanmask = chans;
+        cs.rowmask = masks & 0xffffu;
+        cs.chanmask = masks >> 16;
         cs.gmask = gauss;
     }
 }
