@@ -188,8 +188,14 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
         cs.off[lane + 1] = incl;
         cs.toff[lane + 1] = tincl;
         if (cnt > 0) {
-            const int kind = kind_of(p.r_kind), gi = gidx_of(p.r_kind);
-            const WFactors f = weight_factors(kind, p.h, pick4(d.gauss_t, gi), pick4(d.gauss_inv_norm, gi));
+            const int kind = kind_of(p.r_kind);
+            WFactors f;
+            if (kind == MI_KERNEL_GAUSS) {  // the truncation and norm only for the Gaussian
+                const int gi = gidx_of(p.r_kind);
+                f = weight_factors(kind, p.h, pick4(d.gauss_t, gi), pick4(d.gauss_inv_norm, gi));
+            } else {
+                f = weight_factors(kind, p.h, 0.0f, 0.0f);
+            }
             WalkRec w;
             w.px = (double)p.ix + (double)p.fx;
             w.py = (double)p.iy + (double)p.fy;
