@@ -12,8 +12,8 @@ Everything that needs FP64 happens here, once per particle:
 * the line-of-sight velocity in channel-edge units u = (v - e0)/de split as
   k0 + fu, because channel widths sit near FP32 epsilon of the velocities
   themselves;
-* the channel window outside which the Gaussian line is below erfc(4.5)/2
-  (2e-10) of its flux, or the Dirac line's channel(s) with the reference's
+* the channel window outside which the Gaussian line carries less than
+  erfc(4)/2 (7.7e-9) of its flux on either side, or the Dirac line's channel(s) with the reference's
   inclusive edge test (spectral_models.py:510-570);
 * amp = mHI / d^2 / |dv| / 2.36e5 (spectral_models.py:92-145) times the
   output scale (1/px^2 converts Jy/pix^2 to Jy/arcsec^2, martini.py:364-366).
@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _lib
 
-WINDOW_ERF_ARG = 4.5
+WINDOW_ERF_ARG = 4.0
 MHI_TO_JY = 2.36e5
 _MAX_INDEX = 1 << 21  # |pixel coordinate| bound
 _MAX_RANGE = 1 << 22  # r is stored in 24 bits of r_kind; larger r already covers the cube

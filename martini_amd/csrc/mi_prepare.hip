@@ -33,7 +33,7 @@ constexpr double M_P = 1.67262192369e-27;  // kg
 constexpr double HI_FREQ = 1.420405751e9;  // Hz
 constexpr double C_LIGHT = 299792458.0;    // m/s
 constexpr double MHI_TO_JY = 2.36e5;
-constexpr double WINDOW_ARG = 4.5;         // erf argument of the spectral window edge
+constexpr double WINDOW_ARG = 4.0;         // erf argument of the spectral window edge
 constexpr double MAX_INDEX = 2097152.0;    // 2^21: |pixel coordinate| bound
 constexpr double MAX_RANGE = 4194304.0;    // 2^22: r bound (r_kind has 24 bits for it)
 
