@@ -13,8 +13,9 @@ Everything that needs FP64 happens here, once per particle:
   k0 + fu, because channel widths sit near FP32 epsilon of the velocities
   themselves;
 * the channel window outside which the Gaussian line carries less than
-  erfc(4)/2 (7.7e-9) of its flux on either side, or the Dirac line's channel(s) with the reference's
-  inclusive edge test (spectral_models.py:510-570);
+  erfc(4)/2 (7.7e-9) of its flux on either side, or the Dirac line's
+  channel(s) with the reference's inclusive edge test
+  (spectral_models.py:510-570);
 * amp = mHI / d^2 / |dv| / 2.36e5 (spectral_models.py:92-145) times the
   output scale (1/px^2 converts Jy/pix^2 to Jy/arcsec^2, martini.py:364-366).
 """
