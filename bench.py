@@ -76,7 +76,7 @@ def prepare(workload_name: str):
     return w, m, rec, desc, n_pairs, host
 
 
-def upload(rec, method: str):
+def upload(rec, method: str, nc=None):
     """Records into HBM as the insertion path lays them out (Inserter.upload);
     returns the device records and the host-to-device copy time in ms."""
     import torch
@@ -93,7 +93,7 @@ def upload(rec, method: str):
     d.copy_(host)
     torch.cuda.synchronize()
     h2d_ms = (time.perf_counter() - t0) * 1e3
-    return layout_records(d, method), h2d_ms
+    return layout_records(d, method, nc), h2d_ms
 
 
 def time_gpu(d_rec, n, desc, n_pairs, method: str, steps: int, warmup: int):
@@ -274,7 +274,7 @@ def time_forms(workload: str, methods, steps: int, warmup: int) -> dict:
         f"({time.time() - t:.1f}s with the disc)")
     forms = {}
     for method in methods:
-        d_rec, h2d_ms = upload(rec, method)
+        d_rec, h2d_ms = upload(rec, method, desc.nc)
         ms, gather_ms, bin_ms, st, _ = time_gpu(d_rec, n, desc, n_pairs, method, steps, warmup)
         del d_rec
         torch.cuda.empty_cache()

@@ -24,9 +24,16 @@ Grid make_grid(const mi_cube_desc& d) {
     g.nx = d.nx;
     g.ny = d.ny;
     g.nc = d.nc;
-    g.ntx = (d.nx + TILE - 1) / TILE;
-    g.nty = (d.ny + TILE - 1) / TILE;
-    g.ncb = (d.nc + CBLOCK - 1) / CBLOCK;
+    g.wide = wide_cells(d.nc);
+    g.tx_shift = g.wide ? 3 : 4;
+    g.ty_shift = 4;
+    g.cb_shift = g.wide ? 7 : 6;
+    static_assert(GeoWide::TX == 8 && GeoSquare::TX == 16 && GeoWide::TY == 16 &&
+                      GeoSquare::TY == 16 && GeoWide::CB == 128 && GeoSquare::CB == 64,
+                  "the shifts above");
+    g.ntx = (d.nx + (1 << g.tx_shift) - 1) >> g.tx_shift;
+    g.nty = (d.ny + (1 << g.ty_shift) - 1) >> g.ty_shift;
+    g.ncb = (d.nc + (1 << g.cb_shift) - 1) >> g.cb_shift;
     g.ncells = (int64_t)g.ntx * g.nty * g.ncb;
     int b = 1;
     while (((int64_t)1 << b) < g.ncells) ++b;
@@ -82,7 +89,7 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     L->item_keys2 = take(4 * (size_t)L->items_max);
     L->item_idx = take(4 * (size_t)L->items_max);
     L->item_order = take(4 * (size_t)L->items_max);
-    L->partials = take(sizeof(float) * (size_t)L->multi_max * TILE_PX * CBLOCK);
+    L->partials = take(sizeof(float) * (size_t)L->multi_max * CELL_VOXELS);
     L->status = take(4 * 8);
     // rocPRIM temporary storage: the larger of its two uses (the pair and the item sort).
     size_t t1 = 0, t2 = 0;

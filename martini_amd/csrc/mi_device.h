@@ -18,10 +18,29 @@
 
 namespace mi {
 
-// 16 x 16 pixel tiles, 64-channel blocks: one cell per (tile, block).
-constexpr int TILE = 16;
-constexpr int TILE_PX = TILE * TILE;
-constexpr int CBLOCK = 64;
+// One cell per (pixel tile, channel block).  A tile row (16 pixels along y)
+// is one operand row of the products.  Two shapes, chosen per cube by its
+// channel count (wide_cells): 8 x 16-pixel tiles x 128-channel blocks, so a
+// particle's W is formed once per 128 channels of its line, for cubes of 128
+// channels or more; 16 x 16-pixel tiles x 64-channel blocks below that, where
+// a 128-channel block would be half empty.  Both hold 16384 voxels.
+template <int TX_, int TY_, int CB_>
+struct CellGeo {
+    static constexpr int TX = TX_;  // tile rows (pixels along x)
+    static constexpr int TY = TY_;  // tile columns (pixels along y)
+    static constexpr int PX = TX_ * TY_;
+    static constexpr int CB = CB_;  // channels per block
+};
+using GeoSquare = CellGeo<16, 16, 64>;
+using GeoWide = CellGeo<8, 16, 128>;
+constexpr int CELL_VOXELS = 16384;  // one cell's (partial) tile, either shape
+static_assert(GeoSquare::PX * GeoSquare::CB == CELL_VOXELS &&
+                  GeoWide::PX * GeoWide::CB == CELL_VOXELS,
+              "both cell shapes hold the same number of voxels");
+constexpr int TILE_MAX = 16;  // the longer tile side of either shape
+constexpr int TILE_Y = 16;    // tile columns, the same in both shapes
+static_assert(GeoSquare::TY == TILE_Y && GeoWide::TY == TILE_Y, "one row length");
+__host__ __device__ inline bool wide_cells(int nc) { return nc >= 128; }
 // Particles per cell processed by one work item: longer cells are split and
 // their partial tiles summed in item order by a reduction pass.  The split
 // length depends only on the pair count (about ITEM_TARGET items, between

@@ -2,18 +2,19 @@
 //
 //   cube[i, j, c] += sum_p W[(i, j), p] * S[p, c]
 //
-// over one cell (16 x 16 pixels x 64 channels) per work item, replacing the
+// over one cell (16 x 16 pixels x 64 channels, or 8 x 16 x 128 for cubes of
+// 128 channels or more) per work item, replacing the
 // reference's per-pixel loop (martini/martini.py:243-283, :327-362).
 //
 // A work group of 256 threads takes the item's particles in chunks of 32:
 //   1. the chunk's records go to LDS with their footprints clipped to the tile;
 //   2. W[k][pixel] is evaluated once per (particle, footprint pixel) pair --
 //      pairs are enumerated over the chunk so every lane does useful
-//      transcendental work -- into a zeroed 32 x 256 LDS tile;
-//   3. S[k][channel] (line profile times amplitude) fills a 32 x 64 LDS tile;
+//      transcendental work -- into a zeroed 32 x pixels LDS tile;
+//   3. S[k][channel] (line profile times amplitude) fills a 32 x channels LDS tile;
 //   4. each thread owns one tile row (16 pixels) x 4 channels and accumulates
 //      up to 64 FMAs per particle; a wave skips particles whose footprint
-//      misses its 4 rows, and the 4-column groups the footprint misses.
+//      misses its rows (4 or 2), and the 4-column groups the footprint misses.
 // Summation follows the sorted particle order: no atomics, deterministic.
 #include <hip/hip_runtime.h>
 
@@ -28,6 +29,7 @@ namespace mi {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Three waves per SIMD (at most 168 VGPRs): three work groups share a CU.
+template <class G>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_gather_plain(
     const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
     const Item* __restrict__ items, const uint32_t* __restrict__ item_order,
@@ -39,27 +41,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const CellPos cp = cell_pos(it.cell, g);
     const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
 
-    __shared__ __attribute__((aligned(16))) float Wt[KC][TILE_PX];
-    // rows padded to 68 floats: the S fill writes a column (32 particles) per
-    // store, which then spreads over the LDS banks
-    __shared__ __attribute__((aligned(16))) float St[KC][CBLOCK + 4];
+    __shared__ __attribute__((aligned(16))) float Wt[KC][G::PX];
+    // rows padded to CB + 4 floats: the S fill writes a column (32
+    // particles) per store, which then spreads over the LDS banks
+    __shared__ __attribute__((aligned(16))) float St[KC][G::CB + 4];
     __shared__ ChunkState cs;
     __shared__ float4 Tk[LTAB_N];  // the item's kernel table
     __shared__ GaussTables gt;     // the chunk's Gaussian-kernel factors
 
     const int t = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
-    const int pg = t >> 4;  // tile row (pixel offset along x)
-    const int cg = t & 15;  // 4-channel group
+    constexpr int CG = G::CB / 4;              // 4-channel groups of a block
+    constexpr int WROWS = G::TX / (NT / 64);   // tile rows per wave
+    static_assert(G::TX * CG == NT && WROWS * (NT / 64) == G::TX,
+                  "a thread per (tile row, 4-channel group)");
+    const int pg = t / CG;  // tile row (pixel offset along x)
+    const int cg = t % CG;  // 4-channel group
     const int lane = t & 63;
-    const int wrow0 = x0 + wave * 4, wrow1 = wrow0 + 3;
 
-    for (int e = t; e < KC * TILE_PX / 4; e += NT)
+    for (int e = t; e < KC * G::PX / 4; e += NT)
         reinterpret_cast<float4*>(&Wt[0][0])[e] = make_float4(0.f, 0.f, 0.f, 0.f);
 
-    float acc[TILE][4];
+    float acc[TILE_Y][4];
 #pragma unroll
-    for (int jj = 0; jj < TILE; ++jj)
+    for (int jj = 0; jj < TILE_Y; ++jj)
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) acc[jj][cc] = 0.0f;
 
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         if (lane < min(KC, it.end - kb - KC)) next = load_record(P, next_idx);
         if (lane < min(KC, it.end - kb - 2 * KC)) next_idx = order[kb + 2 * KC + lane];
         __syncthreads();
-        // Particles whose footprint meets each wave's 4 rows (wave-uniform masks);
+        // Particles whose footprint meets each wave's rows (wave-uniform masks);
         // this wave keeps its own, and the least loaded wave stages the next chunk.
         const int klo = cs.ilo[lane & (KC - 1)], khi = cs.ihi[lane & (KC - 1)];
         const bool valid = lane < KC && klo <= khi;
@@ -89,8 +94,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         int fewest = KC + 1;
 #pragma unroll
         for (int w2 = 0; w2 < NT / 64; ++w2) {
-            const int r0 = x0 + w2 * 4;
-            const uint32_t m = (uint32_t)__ballot(valid && khi >= r0 && klo <= r0 + 3);
+            const int r0 = x0 + w2 * WROWS;
+            const uint32_t m = (uint32_t)__ballot(valid && khi >= r0 && klo <= r0 + WROWS - 1);
             const int c = __builtin_popcount(m);
             if (c < fewest) {
                 fewest = c;
@@ -118,7 +123,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 Wt[k][p1] = w1;
                 Wt[k][p2] = w2;
             });
-        fill_spectra(cs, nk, c0, g.nc, d, [&](int k, int c1, float v1, int c2, float v2) {
+        fill_spectra<G::CB>(cs, nk, c0, g.nc, d, [&](int k, int c1, float v1, int c2, float v2) {
             St[k][c1] = v1;
             St[k][c2] = v2;
         });
@@ -128,7 +133,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             const int k = __builtin_ctz(kmask);
             kmask &= kmask - 1u;
             const uint32_t qk = __builtin_amdgcn_readlane(qm, k);  // wave-uniform
-            const f32x4* wr = reinterpret_cast<const f32x4*>(&Wt[k][pg * TILE]);
+            const f32x4* wr = reinterpret_cast<const f32x4*>(&Wt[k][pg * TILE_Y]);
             const f32x4 sv = *reinterpret_cast<const f32x4*>(&St[k][cg * 4]);
             // all five LDS reads are issued together, ahead of the skips
             f32x4 wq[4];
@@ -149,10 +154,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 }
             }
         }
-        // Clear this wave's 64-pixel band of W (read only by this wave) for the next fill.
+        // Clear this wave's band of W (its rows' pixels, read only by this wave)
+        // for the next fill: BAND / 4 lanes per particle, 64 / (BAND / 4) particles
+        // per store.
+        constexpr int BAND = WROWS * TILE_Y;
+        static_assert(BAND % 4 == 0 && 64 % (BAND / 4) == 0, "whole float4 rows per store");
 #pragma unroll
-        for (int k = 0; k < KC; k += 4)
-            *reinterpret_cast<float4*>(&Wt[k + (lane >> 4)][wave * 64 + (lane & 15) * 4]) =
+        for (int k = 0; k < KC; k += 64 / (BAND / 4))
+            *reinterpret_cast<float4*>(
+                &Wt[k + lane / (BAND / 4)][wave * BAND + (lane % (BAND / 4)) * 4]) =
                 make_float4(0.f, 0.f, 0.f, 0.f);
     }
 
@@ -160,7 +170,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     if (it.slot < 0) {
         if (i < g.nx) {
 #pragma unroll
-            for (int jj = 0; jj < TILE; ++jj) {
+            for (int jj = 0; jj < TILE_Y; ++jj) {
                 const int j = y0 + jj;
                 if (j >= g.ny) continue;
                 float* dst = cube + ((size_t)i * g.ny + j) * g.nc;
@@ -172,19 +182,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             }
         }
     } else {
-        float4* dst = reinterpret_cast<float4*>(partials + (size_t)it.slot * TILE_PX * CBLOCK);
+        float4* dst = reinterpret_cast<float4*>(partials + (size_t)it.slot * CELL_VOXELS);
 #pragma unroll
-        for (int jj = 0; jj < TILE; ++jj)
-            dst[(pg * TILE + jj) * (CBLOCK / 4) + cg] =
+        for (int jj = 0; jj < TILE_Y; ++jj)
+            dst[(pg * TILE_Y + jj) * CG + cg] =
                 make_float4(acc[jj][0], acc[jj][1], acc[jj][2], acc[jj][3]);
     }
 }
 
 // Sums the partial tiles of each split cell in item order and adds them to
 // the cube.  Block (cell, part): 256 threads x 4 voxels cover a 1024-voxel
-// part of the 16384-voxel tile; the item loop is unrolled so its loads are
+// part of the CELL_VOXELS = 16384-voxel tile; the item loop is unrolled so its loads are
 // independent.
-constexpr int RED_PARTS = TILE_PX * CBLOCK / (NT * 4);
+constexpr int RED_PARTS = CELL_VOXELS / (NT * 4);
 
 __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item_off,
                                                const uint32_t* __restrict__ multi_off,
@@ -196,7 +206,7 @@ __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item
     const int part = blockIdx.x - cell * RED_PARTS;
     const uint32_t ni = item_off[cell + 1] - item_off[cell];
     if (ni <= 1) return;
-    const size_t tile = (size_t)TILE_PX * CBLOCK / 4;  // float4 per partial tile
+    const size_t tile = (size_t)CELL_VOXELS / 4;  // float4 per partial tile
     const float4* base = reinterpret_cast<const float4*>(partials) + (size_t)multi_off[cell] * tile;
     const int v4 = part * NT + threadIdx.x;  // float4 index within the tile
     float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -214,8 +224,8 @@ __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item
         s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
     }
     const CellPos cp = cell_pos(cell, g);
-    const int px = (v4 * 4) / CBLOCK, cl = (v4 * 4) % CBLOCK;
-    const int i = cp.x0 + px / TILE, j = cp.y0 + px % TILE;
+    const int px = (v4 * 4) >> g.cb_shift, cl = (v4 * 4) & ((1 << g.cb_shift) - 1);
+    const int i = cp.x0 + px / TILE_Y, j = cp.y0 + px % TILE_Y;
     if (i >= g.nx || j >= g.ny) return;
     float* dst = cube + ((size_t)i * g.ny + j) * g.nc + cp.c0 + cl;
     const float sv[4] = {s0.x, s0.y, s0.z, s0.w};
@@ -279,11 +289,16 @@ hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_des
     const float4* wtab = nullptr;
     hipError_t e = weight_table(s, &wtab);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gather_plain, dim3((unsigned)L.items_max), dim3(NT), 0, s,
-                       d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
-                       (const uint32_t*)(ws + L.item_order),
-                       (const uint32_t*)(ws + L.status), wtab, g, d,
-                       d_cube, (float*)(ws + L.partials));
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)L.items_max), dim3(NT), 0, s, d_p,
+                           (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
+                           (const uint32_t*)(ws + L.item_order), (const uint32_t*)(ws + L.status),
+                           wtab, g, d, d_cube, (float*)(ws + L.partials));
+    };
+    if (g.wide)
+        launch(k_gather_plain<GeoWide>);
+    else
+        launch(k_gather_plain<GeoSquare>);
     return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@ namespace mi {
 
 struct Grid {
     int nx, ny, nc;      // padded cube extent
+    bool wide;           // cell shape: GeoWide, else GeoSquare (wide_cells)
+    int tx_shift, ty_shift, cb_shift;  // log2 of the tile sides and the block
     int ntx, nty, ncb;   // tiles along x, y; channel blocks
     int64_t ncells;
     int cell_bits;       // bits needed for a cell id
@@ -28,12 +30,12 @@ __host__ __device__ inline uint32_t cells_of(const mi_particle& p, const Grid& g
     int c_lo = p.c_lo < 0 ? 0 : p.c_lo;
     int c_hi = p.c_hi > g.nc - 1 ? g.nc - 1 : p.c_hi;
     if (c_lo > c_hi) return 0;
-    tx0 = i_lo / TILE;
-    tx1 = i_hi / TILE;
-    ty0 = j_lo / TILE;
-    ty1 = j_hi / TILE;
-    cb0 = c_lo / CBLOCK;
-    cb1 = c_hi / CBLOCK;
+    tx0 = i_lo >> g.tx_shift;  // footprints and windows are clipped: non-negative
+    tx1 = i_hi >> g.tx_shift;
+    ty0 = j_lo >> g.ty_shift;
+    ty1 = j_hi >> g.ty_shift;
+    cb0 = c_lo >> g.cb_shift;
+    cb1 = c_hi >> g.cb_shift;
     return (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1) * (cb1 - cb0 + 1));
 }
 
@@ -48,7 +50,7 @@ struct Layout {
     size_t items;                      // int4[items_max], in cell order
     size_t item_keys, item_keys2;      // uint32[items_max]: sort keys of the launch order
     size_t item_idx, item_order;       // uint32[items_max]: item index; launch order
-    size_t partials;                   // float[multi_max * TILE_PX * CBLOCK]
+    size_t partials;                   // float[multi_max * CELL_VOXELS]
     size_t status;                     // uint32[8]
     size_t temp;                       // rocPRIM temporary storage
     size_t temp_bytes;

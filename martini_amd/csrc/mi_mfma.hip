@@ -2,18 +2,22 @@
 //
 //   cube[(i, j), c] += sum_p W[(i, j), p] * S[p, c]
 //
-// per cell (16 x 16 pixels x 64 channels) as a block-sparse product of a
-// 256 x 32 W tile and a 32 x 64 S tile per chunk of 32 spatially sorted
-// particles.  Both operands are split into BF16 pairs x = hi + lo
+// per cell (16 x 16 pixels x 64 channels, or 8 x 16 x 128 for cubes of 128
+// channels or more) as a block-sparse product of a pixels x 32 W tile and a
+// 32 x channels S tile per chunk of 32 spatially sorted particles.  Both operands are split into BF16 pairs x = hi + lo
 // (|x - hi - lo| <= 2^-16 |x|) when they are written to LDS, and each
 // 16 x 16 x 32 block takes three v_mfma_f32_16x16x32_bf16 products -- hi*hi,
 // hi*lo, lo*hi -- accumulated in FP32 (the dropped lo*lo term is below 2^-16
 // of the product).
 //
-// Wave w owns tile rows 4w..4w+3 (M = the 16 pixels of a row) and all four
-// 16-channel blocks (N).  A (row, channel-block) product is skipped when no
-// footprint of the chunk touches the row or no spectral window touches the
-// block; the masks are wave-uniform, so the skips are scalar branches.
+// Wave w owns four tile rows (M = the 16 pixels of a row) and four 16-channel
+// blocks (N): with 16 x 16-pixel x 64-channel cells the rows 4w..4w+3 and all
+// four blocks; with 8 x 16 x 128 cells rows 4 (w % 2) .. + 3 and blocks
+// 4 (w / 2) .. + 3, each pair of waves sharing its rows, which the second of
+// the two to finish reading them clears (an LDS counter per row group decides).  A (row, channel-block) product is
+// skipped when no footprint of the chunk touches the row or no spectral window
+// touches the block; the masks are wave-uniform, so the skips are scalar
+// branches.
 //
 // LDS operand tiles hold 32 BF16 values (64 bytes) per pixel or channel, k
 // contiguous, with the four 16-byte k-groups of a row permuted by bits 2..3 of
@@ -72,6 +76,24 @@ __device__ inline void put_split2(char* hi, char* lo, int k, int row1, float x1,
     *reinterpret_cast<__bf16*>(lo + o2) = l.y;
 }
 
+// put_split2 for rows given as 16 * block + r: the row's byte offset is the
+// block's 1 KiB step plus op_offset(r), so the block becomes a store's
+// immediate offset and the per-r offsets are shared by all blocks.
+__device__ inline void put_split2_blocks(char* hi, char* lo, int k, int row1, float x1, int row2,
+                                         float x2) {
+    const floatx2 x = {x1, x2};
+    const bf16x2 h = __builtin_convertvector(x, bf16x2);
+    const floatx2 hf = __builtin_convertvector(h, floatx2);
+    const bf16x2 l = __builtin_convertvector(floatx2{x1 - hf.x, x2 - hf.y}, bf16x2);
+    const int kb = k_bits(k);
+    const int o1 = (row1 >> 4) * (16 * KROW) + op_offset(row1 & 15, kb);
+    const int o2 = (row2 >> 4) * (16 * KROW) + op_offset(row2 & 15, kb);
+    *reinterpret_cast<__bf16*>(hi + o1) = h.x;
+    *reinterpret_cast<__bf16*>(lo + o1) = l.x;
+    *reinterpret_cast<__bf16*>(hi + o2) = h.y;
+    *reinterpret_cast<__bf16*>(lo + o2) = l.y;
+}
+
 // The 8 BF16 values of k-group kg of `row`.
 __device__ inline bf16x8 get8(const char* tile, int row, int kg) {
     return *reinterpret_cast<const bf16x8*>(tile + row * KROW + (((kg ^ (row >> 2)) & 3) << 4));
@@ -79,6 +101,7 @@ __device__ inline bf16x8 get8(const char* tile, int row, int kg) {
 
 // Three waves per SIMD: the 64 accumulators then live in VGPRs and three work
 // groups share a CU (LDS 48 KB each).
+template <class G>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_gather_mfma(
     const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
     const Item* __restrict__ items, const uint32_t* __restrict__ item_order,
@@ -90,13 +113,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const CellPos cp = cell_pos(it.cell, g);
     const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
 
-    __shared__ __attribute__((aligned(16))) char Ah[TILE_PX * KROW];  // W hi [pixel][k]
-    __shared__ __attribute__((aligned(16))) char Al[TILE_PX * KROW];  // W lo
-    __shared__ __attribute__((aligned(16))) char Bh[CBLOCK * KROW];   // S hi [channel][k]
-    __shared__ __attribute__((aligned(16))) char Bl[CBLOCK * KROW];   // S lo
+    __shared__ __attribute__((aligned(16))) char Ah[G::PX * KROW];  // W hi [pixel][k]
+    __shared__ __attribute__((aligned(16))) char Al[G::PX * KROW];  // W lo
+    __shared__ __attribute__((aligned(16))) char Bh[G::CB * KROW];  // S hi [channel][k]
+    __shared__ __attribute__((aligned(16))) char Bl[G::CB * KROW];  // S lo
     __shared__ ChunkState cs;
     __shared__ float4 Tk[LTAB_N];  // the item's kernel table
     __shared__ GaussTables gt;     // the chunk's Gaussian-kernel factors
+    // Waves: RG groups of four tile rows x CGN groups of four 16-channel blocks.
+    constexpr int RG = G::TX / 4, CGN = G::CB / 64;
+    static_assert(RG * CGN == NT / 64, "one (row group, channel group) per wave");
+    __shared__ uint32_t done_rows[RG];  // waves done reading each row group this chunk
 
     const int t = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform
@@ -104,10 +131,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int m = lane & 15;   // A row / B column / D column of this lane
     const int kg = lane >> 4;  // K group of 8 (A, B); D row group of 4
 
-    for (int e = t; e < TILE_PX * KROW / 16; e += NT) {
+    for (int e = t; e < G::PX * KROW / 16; e += NT) {
         reinterpret_cast<uint4*>(Ah)[e] = make_uint4(0u, 0u, 0u, 0u);
         reinterpret_cast<uint4*>(Al)[e] = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (t < RG) done_rows[t] = 0u;
+    const int rh = wave % RG, ch = wave / RG;  // this wave's row group and channel group
 
     floatx4 acc[4][4];
 #pragma unroll
@@ -133,12 +162,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         if (lane < min(KC, it.end - kb - 2 * KC)) next_idx = order[kb + 2 * KC + lane];
         __syncthreads();
         const uint32_t rowmask = __builtin_amdgcn_readfirstlane(cs.rowmask);
-        const uint32_t rows = (rowmask >> (wave * 4)) & 15u;
-        const uint32_t chans = __builtin_amdgcn_readfirstlane(cs.chanmask);
-        int fewest = 5;
+        const uint32_t chanmask = __builtin_amdgcn_readfirstlane(cs.chanmask);
+        const uint32_t rows = (rowmask >> (rh * 4)) & 15u;
+        const uint32_t chans = (chanmask >> (ch * 4)) & 15u;
+        int fewest = 17;
 #pragma unroll
         for (int w2 = 0; w2 < NT / 64; ++w2) {
-            const int c = __builtin_popcount((rowmask >> (w2 * 4)) & 15u);
+            const int c = __builtin_popcount((rowmask >> ((w2 % RG) * 4)) & 15u) *
+                          __builtin_popcount((chanmask >> ((w2 / RG) * 4)) & 15u);
             if (c < fewest) {
                 fewest = c;
                 stager = w2;
@@ -153,11 +184,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                      [&](int k, int p1, float w1, int p2, float w2) {
                          put_split2(Ah, Al, k, p1, w1, p2, w2);
                      });
-        fill_spectra(cs, nk, c0, g.nc, d,
+        fill_spectra<G::CB>(cs, nk, c0, g.nc, d,
                      [&](int k, int c1, float s1, int c2, float s2) {
-                         put_split2(Bh, Bl, k, c1, s1, c2, s2);
+                         put_split2_blocks(Bh, Bl, k, c1, s1, c2, s2);
                      },
-                     chans);
+                     chanmask);
         __syncthreads();
         if (rows != 0u) {
             // The wave's four tile rows held together: each S operand read serves all.
@@ -166,15 +197,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             for (int r = 0; r < 4; ++r) {
                 ah[r] = bf16x8{};
                 al[r] = bf16x8{};
-                if (!(rows & (1u << r))) continue;
-                const int row = (wave * 4 + r) * TILE + m;
+                if (!(rows & (1u << r)) || chans == 0u) continue;
+                const int row = (rh * 4 + r) * TILE_Y + m;
                 ah[r] = get8(Ah, row, kg);
                 al[r] = get8(Al, row, kg);
             }
 #pragma unroll
             for (int nb = 0; nb < 4; ++nb) {
                 if (!(chans & (1u << nb))) continue;
-                const bf16x8 bh = get8(Bh, nb * 16 + m, kg), bl = get8(Bl, nb * 16 + m, kg);
+                const int col = (ch * 4 + nb) * 16 + m;
+                const bf16x8 bh = get8(Bh, col, kg), bl = get8(Bl, col, kg);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (!(rows & (1u << r))) continue;
@@ -185,14 +217,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     acc[r][nb] = c;
                 }
             }
-            // Clear the rows' 1 KiB in both W tiles (one 16-byte store per lane
-            // each); the reads above come first in this wave's LDS order.
+            // The last of the CGN waves sharing these rows to get here clears
+            // the rows' 1 KiB in both W tiles (one 16-byte store per lane
+            // each): the counter's update orders it after every such wave's
+            // reads of them (LDS requests of a wave are served in order; the
+            // release keeps the reads before the update).
+            uint32_t before = 0u;
+            if (CGN > 1 && lane == 0)
+                before = __hip_atomic_fetch_add(&done_rows[rh], 1u, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (CGN == 1 || __builtin_amdgcn_readlane(before, 0) == CGN - 1) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (!(rows & (1u << r))) continue;
-                const int zb = (wave * 4 + r) * TILE * KROW + lane * 16;
-                *reinterpret_cast<uint4*>(Ah + zb) = make_uint4(0u, 0u, 0u, 0u);
-                *reinterpret_cast<uint4*>(Al + zb) = make_uint4(0u, 0u, 0u, 0u);
+                for (int r = 0; r < 4; ++r) {
+                    if (!(rows & (1u << r))) continue;
+                    const int zb = (rh * 4 + r) * TILE_Y * KROW + lane * 16;
+                    *reinterpret_cast<uint4*>(Ah + zb) = make_uint4(0u, 0u, 0u, 0u);
+                    *reinterpret_cast<uint4*>(Al + zb) = make_uint4(0u, 0u, 0u, 0u);
+                }
+                if (CGN > 1 && lane == 0) done_rows[rh] = 0u;
             }
         }
     }
@@ -200,20 +242,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // D layout of a 16x16 block: lane holds rows 4*kg..4*kg+3 (pixels j) of column m (channel).
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int prow = wave * 4 + r;  // tile row
+        const int prow = rh * 4 + r;  // tile row
         const int i = x0 + prow;
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int jl = kg * 4 + q;
-                const int cl = nb * 16 + m;
+                const int cl = (ch * 4 + nb) * 16 + m;
                 const float v = acc[r][nb][q];
                 if (it.slot < 0) {
                     const int j = y0 + jl, c = c0 + cl;
                     if (i < g.nx && j < g.ny && c < g.nc) cube[((size_t)i * g.ny + j) * g.nc + c] += v;
                 } else {
-                    partials[(size_t)it.slot * TILE_PX * CBLOCK + (prow * TILE + jl) * CBLOCK + cl] = v;
+                    partials[(size_t)it.slot * CELL_VOXELS + (prow * TILE_Y + jl) * G::CB + cl] = v;
                 }
             }
         }
@@ -226,11 +268,16 @@ hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc
     const float4* wtab = nullptr;
     hipError_t e = weight_table(s, &wtab);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gather_mfma, dim3((unsigned)L.items_max), dim3(NT), 0, s,
-                       d_p, (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
-                       (const uint32_t*)(ws + L.item_order),
-                       (const uint32_t*)(ws + L.status), wtab, g, d,
-                       d_cube, (float*)(ws + L.partials));
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)L.items_max), dim3(NT), 0, s, d_p,
+                           (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
+                           (const uint32_t*)(ws + L.item_order), (const uint32_t*)(ws + L.status),
+                           wtab, g, d, d_cube, (float*)(ws + L.partials));
+    };
+    if (g.wide)
+        launch(k_gather_mfma<GeoWide>);
+    else
+        launch(k_gather_mfma<GeoSquare>);
     return hipGetLastError();
 }
 

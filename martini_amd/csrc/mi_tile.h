@@ -76,8 +76,8 @@ struct ChunkState {
 // ex and ey).  Filled once per chunk, they leave one erf and the FP64
 // truncation term per pixel.
 struct GaussTables {
-    float ex[KC][TILE];
-    float ey[KC][TILE];
+    float ex[KC][TILE_MAX];
+    float ey[KC][TILE_MAX];
 };
 
 // Wave-wide inclusive scans with DPP (row shifts within 16-lane rows, then the
@@ -158,17 +158,17 @@ __device__ inline void stage_chunk(const RecRegs& r, bool valid, int x0, int y0,
     // lanes without a particle or a footprint in the tile end with cnt = 0.
     int fi_lo = 0, fi_hi = -1, fj_lo = 0, fj_hi = -1;
     const bool fp = footprint(p, g.nx, g.ny, fi_lo, fi_hi, fj_lo, fj_hi);
-    const int ci_lo = max(fi_lo, x0), ci_hi = min(fi_hi, x0 + TILE - 1);
-    const int cj_lo = max(fj_lo, y0), cj_hi = min(fj_hi, y0 + TILE - 1);
+    const int ci_lo = max(fi_lo, x0), ci_hi = min(fi_hi, x0 + (1 << g.tx_shift) - 1);
+    const int cj_lo = max(fj_lo, y0), cj_hi = min(fj_hi, y0 + TILE_Y - 1);
     const bool ok = valid && fp && ci_lo <= ci_hi && cj_lo <= cj_hi;
     const int ilo = ok ? ci_lo : 1, ihi = ok ? ci_hi : 0;
     const int jlo = ok ? cj_lo : 1, jhi = ok ? cj_hi : 0;
     const int cnt = ok ? (ihi - ilo + 1) * (jhi - jlo + 1) : 0;
     const uint32_t rows = ok ? (2u << (ihi - x0)) - (1u << (ilo - x0)) : 0u;
-    const int ca = max(p.c_lo, c0) - c0, cb = min(p.c_hi, c0 + CBLOCK - 1) - c0;
+    const int ca = max(p.c_lo, c0) - c0, cb = min(p.c_hi, c0 + (1 << g.cb_shift) - 1) - c0;
     const uint32_t chans = ok && ca <= cb ? (2u << (cb >> 4)) - (1u << (ca >> 4)) : 0u;
     const int incl = wave_scan_add(cnt);
-    static_assert(TILE <= 16 && CBLOCK / 16 <= 16, "rows and channel blocks share one word");
+    static_assert(TILE_MAX <= 16 && GeoWide::CB / 16 <= 16, "rows and channel blocks share one word");
     const uint32_t masks = wave_or(rows | (chans << 16));  // one reduction for both
     const uint32_t gauss = (uint32_t)__ballot(lane < KC && cnt > 0 &&
                                               kind_of(p.r_kind) == MI_KERNEL_GAUSS);
@@ -306,7 +306,7 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
     const int l0 = (t - cs.toff[k]) * cs.per;
     const int l1 = active ? min(l0 + cs.per, cs.off[k + 1] - cs.off[k]) : l0;
     // l0 / width: (l0 + 1/2) / width lies at least 1/32 from an integer (l0 <
-    // 256, width <= 16), far beyond the reciprocal's error
+    // 256 pixels, width <= 16), far beyond the reciprocal's error
     int a = (int)(((float)l0 + 0.5f) * __builtin_amdgcn_rcpf((float)width));
     int b = l0 - a * width;
     const int ti = (w.ij & 0xffff) - x0, tj = (w.ij >> 16) - y0;  // footprint corner in the tile
@@ -321,15 +321,15 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
         // Pixel l of the footprint lies in row floor((l + 1/2) / width) (exact, as
         // for a above) and column l - row * width, all small integers formed in
         // FP32: no wrap tests, and the offsets from the particle need no
-        // conversion.  Its index in the tile is row * TILE + column + corner.
+        // conversion.  Its index in the tile is row * TILE_Y + column + corner.
         const float rw = __builtin_amdgcn_rcpf((float)width), wf = (float)width;
-        const float hrw = 0.5f * rw, corner4 = (float)(4 * (ti * TILE + tj));
+        const float hrw = 0.5f * rw, corner4 = (float)(4 * (ti * TILE_Y + tj));
         auto at = [&](float lf, float& af, float& bf) {
             af = floorf(fmaf(lf, rw, hrw));  // (l + 1/2) / width, one rounding
             bf = fmaf(-af, wf, lf);
         };
         auto px_of = [&](float af, float bf) {
-            const int q = (int)fmaf(af, (float)(4 * TILE), fmaf(bf, 4.0f, corner4));  // 4 * index
+            const int q = (int)fmaf(af, (float)(4 * TILE_Y), fmaf(bf, 4.0f, corner4));  // 4 * index
             __builtin_assume((q & 3) == 0);
             return q >> 2;
         };
@@ -366,7 +366,7 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
         const TableWalk tw = table_walk(w);
         for (int l = l0; l < l1; ++l) {
             const int i = x0 + ti + a, j = y0 + tj + b;
-            store(k, (ti + a) * TILE + tj + b, uw[tw.base + (int64_t)i * tw.stride + j]);
+            store(k, (ti + a) * TILE_Y + tj + b, uw[tw.base + (int64_t)i * tw.stride + j]);
             if (++b == width) {
                 b = 0;
                 ++a;
@@ -380,7 +380,7 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
         const float4 gg = cs.gr[k].f;
         for (int l = l0; l < l1; ++l) {
             const int i = x0 + ti + a, j = y0 + tj + b;
-            store(k, (ti + a) * TILE + tj + b,
+            store(k, (ti + a) * TILE_Y + tj + b,
                   gauss_weight(w.px - (double)i, w.py - (double)j, w.fa, fe, gg.y, gg.z,
                                gt.ex[k][a], gt.ey[k][b]));
             if (++b == width) {
@@ -393,7 +393,7 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
     const WFactors f{w.fa, 0.0, (float)w.fa, 0.0f, 0.0f};
     for (int l = l0; l < l1; ++l) {
         const int i = x0 + ti + a, j = y0 + tj + b;
-        store(k, (ti + a) * TILE + tj + b,
+        store(k, (ti + a) * TILE_Y + tj + b,
               weight_at(kind, w.px - (double)i, w.py - (double)j, f, wtab, tab_kind, tab));
         if (++b == width) {
             b = 0;
@@ -402,7 +402,7 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
     }
 }
 
-// Evaluates S for the chunk's KC particles x CBLOCK channels of the cell and
+// Evaluates S for the chunk's KC particles x CB channels of the cell and
 // hands the values to store2(k, channel1, s1, channel2, s2), two channels of
 // one particle at a time; particles k >= nk and channels beyond the cube give
 // 0.  Thread t takes particle t % KC and the channels t / KC + 8 i, so its
@@ -412,10 +412,10 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
 // expanded, to FP32 rounding), computed as straight-line code (the window
 // test is one unsigned compare and a select, the wide-channel erfc path one
 // branch per block) so that the evaluations of a thread overlap.
-template <class Store2>
+template <int CB, class Store2>
 __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc,
                                     const mi_cube_desc& d, Store2 store2,
-                                    uint32_t blocks = 0xFu) {
+                                    uint32_t blocks = (1u << (CB / 16)) - 1u) {
     constexpr int PER_BLOCK = 16 / (NT / KC);  // a thread's channels in one 16-channel block
     static_assert(PER_BLOCK == 2, "a thread stores its two channels of a block together");
     const int k = threadIdx.x & (KC - 1);
@@ -438,18 +438,20 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     };
     if (spectrum == MI_SPECTRUM_TABLE) {  // uniform: the caller's spectra
 #pragma unroll
-        for (int b = 0; b < CBLOCK / 16; ++b) {
+        for (int b = 0; b < CB / 16; ++b) {
             if (!((blocks >> b) & 1u)) continue;
-            const int c = c0 + cl0 + (NT / KC) * (b * PER_BLOCK);
-            store2(k, c - c0, table_flux(lo, hi, k0, amp, c, d.spectra, d.spectra_ld),
-                   c - c0 + NT / KC,
-                   table_flux(lo, hi, k0, amp, c + NT / KC, d.spectra, d.spectra_ld));
+            // the row of the caller's table from the thread's first channel on
+            // (channel c0 + cl0 + o is inside the window when inside(o))
+            const float* row = d.spectra + ((int64_t)k0 * d.spectra_ld + c0 + cl0);
+            const int o0 = (NT / KC) * (b * PER_BLOCK), o1 = o0 + NT / KC;
+            store2(k, cl0 + o0, inside(o0) ? amp * row[o0] : 0.0f, cl0 + o1,
+                   inside(o1) ? amp * row[o1] : 0.0f);
         }
         return;
     }
     if (spectrum == MI_SPECTRUM_DIRAC) {  // uniform: the whole amplitude in the window
 #pragma unroll
-        for (int b = 0; b < CBLOCK / 16; ++b)
+        for (int b = 0; b < CB / 16; ++b)
             if ((blocks >> b) & 1u) put(b, amp, amp);
         return;
     }
@@ -464,9 +466,10 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     // Channel c = c0 + cl0 + 8 j of this thread sits at k0 - c + fu - 1/2 =
     // A - 8 j from the line centre (in channels), A formed once; subtracting
     // the small integer 8 j from A is exact in the in-window range.
-    const float A = (float)(k0 - c0 - cl0) + (fu - 0.5f);
+    const int kc = k0 - c0 - cl0;
+    const float A = (float)kc + (fu - 0.5f);
 #pragma unroll
-    for (int b = 0; b < CBLOCK / 16; ++b) {
+    for (int b = 0; b < CB / 16; ++b) {
         if (!((blocks >> b) & 1u)) continue;  // uniform
         float v[PER_BLOCK];
 #pragma unroll
@@ -481,8 +484,7 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
             for (int i = 0; i < PER_BLOCK; ++i) {
                 const int o = (NT / KC) * (b * PER_BLOCK + i);
                 if (!inside(o)) continue;
-                const int c = c0 + cl0 + o;
-                const float m = s * (((float)(k0 - c) + fu) - 0.5f);
+                const float m = s * (((float)(kc - o) + fu) - 0.5f);  // kc - o = k0 - channel
                 v[i] = 0.5f * amp * erf_diff(m - 0.5f * s, m + 0.5f * s);
             }
         }
@@ -514,7 +516,7 @@ struct CellPos {
 __device__ inline CellPos cell_pos(int cell, const Grid& g) {
     const int cb = cell % g.ncb;
     const int txy = cell / g.ncb;
-    return CellPos{(txy / g.nty) * TILE, (txy % g.nty) * TILE, cb * CBLOCK};
+    return CellPos{(txy / g.nty) << g.tx_shift, (txy % g.nty) << g.ty_shift, cb << g.cb_shift};
 }
 
 }  // namespace mi

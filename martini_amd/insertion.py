@@ -36,11 +36,12 @@ class Inserter:
             self._ws = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def upload(self, records: np.ndarray, method="plain"):
+    def upload(self, records: np.ndarray, method="plain", nc=None):
         """Copy host records into HBM (uint8 tensor of n * 48 bytes), laid out
-        for `method` (``layout_records``)."""
+        for `method` and a cube of `nc` channels (``layout_records``)."""
         rec = np.ascontiguousarray(records)
-        return layout_records(self.torch.from_numpy(rec.view(np.uint8)).to(self.device), method)
+        return layout_records(self.torch.from_numpy(rec.view(np.uint8)).to(self.device), method,
+                              nc)
 
     def insert_device(self, d_records, n: int, n_pairs: int, desc, d_cube, method="plain",
                       stream=None) -> None:
@@ -71,7 +72,7 @@ class Inserter:
             desc.weights = d_weights.data_ptr() if host_w.size else None
             desc.weights_n = int(host_w.size)
         n_pairs = _lib.count_pairs(records, desc)
-        d_rec = self.upload(records, method)
+        d_rec = self.upload(records, method, desc.nc)
         if d_cube is None:
             d_cube = torch.zeros(desc.nx * desc.ny * desc.nc, dtype=torch.float32, device=self.device)
         self.insert_device(d_rec, n, n_pairs, desc, d_cube, method=method)
@@ -149,10 +150,18 @@ def _spread4(v):
     return (v & 1) | ((v & 2) << 1) | ((v & 4) << 2) | ((v & 8) << 3)
 
 
-def spatial_records(d_rec):
-    """Records (uint8 tensor of n * 48 bytes) sorted by 16 x 16-pixel tile of
-    their pixel (ix, iy), tiles in the binning's cell order (x-major), and by
-    the pixel's Z order inside the tile.  The MFMA form's binning keeps the
+def cell_tile(nc):
+    """The binning's pixel tile (rows along x, columns along y) for a cube of
+    `nc` channels (mi_device.h wide_cells): 8 x 16 from 128 channels on, where
+    cells take 128-channel blocks, else 16 x 16."""
+    return (8, 16) if nc is not None and nc >= 128 else (16, 16)
+
+
+def spatial_records(d_rec, nc=None):
+    """Records (uint8 tensor of n * 48 bytes) sorted by the binning's pixel tile
+    (``cell_tile(nc)``) of their pixel (ix, iy), tiles in its cell order
+    (x-major), and by the pixel's Z order inside the tile (x and y bits
+    interleaved, x on the even bits).  The MFMA form's binning keeps the
     record order within each cell, so its 32-particle chunks are then compact
     patches of the tile, and a chunk reads neighbouring records, whole cache
     lines instead of one 48-byte record per line.  Equal keys keep the given order (a stable sort),
@@ -166,13 +175,15 @@ def spatial_records(d_rec):
     ix = words[:, 0].to(torch.int64)
     iy = words[:, 1].to(torch.int64)
     off = 1 << 17  # |ix|, |iy| < 2^21: tile indices shifted to be non-negative
-    key = ((((ix >> 4) + off) << 18) | ((iy >> 4) + off)) << 8
-    key = key | _spread4(ix) | (_spread4(iy) << 1)
+    tx, ty = cell_tile(nc)
+    sx, sy = tx.bit_length() - 1, ty.bit_length() - 1
+    key = ((((ix >> sx) + off) << 18) | ((iy >> sy) + off)) << 8
+    key = key | _spread4(ix & (tx - 1)) | (_spread4(iy & (ty - 1)) << 1)
     idx = torch.argsort(key, stable=True)
     return words.index_select(0, idx).view(-1).view(torch.uint8)
 
 
-def layout_records(d_rec, method="plain"):
+def layout_records(d_rec, method="plain", nc=None):
     """The records' HBM layout for a gather method: shuffled for the FP32 form
     (``shuffle_records``: its chunks should spread over the tile), in cell and
     pixel Z order for the MFMA form (``spatial_records``: its chunks are
@@ -180,7 +191,7 @@ def layout_records(d_rec, method="plain"):
     depends on the layout only through the FP32 summation order."""
     if _lib.METHODS[method] == _lib.METHOD_PLAIN_F32:
         return shuffle_records(d_rec)
-    return spatial_records(d_rec)
+    return spatial_records(d_rec, nc)
 
 
 _default = {}

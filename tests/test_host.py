@@ -203,7 +203,7 @@ def _footprints(rec, nx, ny):
     return rr, ilo, ihi, jlo, jhi, cut
 
 
-@pytest.mark.parametrize("name", ["disc1e4_64px", "hetero3e6_1024px"])
+@pytest.mark.parametrize("name", ["disc1e4_64px", "hetero3e6_1024px", "disc1e6_256px"])
 def test_pair_count_matches_numpy(name):
     w = synthetic.WORKLOADS[name]
     m, rec, desc = _records(w, None if w.n <= 10_000 else 30_000)
@@ -211,7 +211,10 @@ def test_pair_count_matches_numpy(name):
     assert cut.mean() > 0.5  # the cut applies to most particles here
     clo, chi = np.maximum(rec["c_lo"], 0), np.minimum(rec["c_hi"], desc.nc - 1)
     ok = (rr >= 0) & (ilo <= ihi) & (jlo <= jhi) & (clo <= chi)
-    n = ((ihi // 16 - ilo // 16 + 1) * (jhi // 16 - jlo // 16 + 1) * (chi // 64 - clo // 64 + 1))
+    # cells (mi_device.h): 8 x 16-pixel tiles x 128-channel blocks from 128
+    # channels on, else 16 x 16 x 64
+    tx, ty, cb = (8, 16, 128) if desc.nc >= 128 else (16, 16, 64)
+    n = ((ihi // tx - ilo // tx + 1) * (jhi // ty - jlo // ty + 1) * (chi // cb - clo // cb + 1))
     assert _lib.count_pairs(rec, desc) == int(n[ok].sum())
 
 

@@ -31,27 +31,29 @@ def test_record_permutation_is_the_hash_order(n):
 
 
 def _morton(x, y):
-    """Z order of a pixel (x, y) in a 16 x 16 tile, bit by bit: x on even bits."""
+    """Z order of a pixel (x, y) in a tile of up to 16 x 16, bit by bit: x on even bits."""
     z = 0
     for b in range(4):
         z |= ((x >> b) & 1) << (2 * b) | ((y >> b) & 1) << (2 * b + 1)
     return z
 
 
-def test_mfma_layout_is_tile_then_z_order():
-    """The MFMA form's records are sorted by 16 x 16 tile (x-major) and by the
-    Z order of the pixel inside it, ties in the given order: checked against a
-    plain Python sort of the same keys, negative pixels included."""
+@pytest.mark.parametrize("nc,tx", [(64, 16), (128, 8), (None, 16)])
+def test_mfma_layout_is_tile_then_z_order(nc, tx):
+    """The MFMA form's records are sorted by the binning's tile (x-major; 16 x 16
+    pixels below 128 channels, 8 x 16 from 128 on) and by the Z order of the
+    pixel inside it, ties in the given order: checked against a plain Python
+    sort of the same keys, negative pixels included."""
     rng = np.random.default_rng(9)
     n = 5000
     rec = np.zeros(n, dtype=_lib.PARTICLE_DTYPE)
     rec["ix"] = rng.integers(-40, 300, n)
     rec["iy"] = rng.integers(-40, 300, n)
     rec["k0"] = np.arange(n)  # the given position
-    out = layout_records(torch.from_numpy(rec.view(np.uint8).copy()), "mfma").numpy()
+    out = layout_records(torch.from_numpy(rec.view(np.uint8).copy()), "mfma", nc).numpy()
     got = out.view(_lib.PARTICLE_DTYPE)
-    want = sorted(range(n), key=lambda i: (int(rec["ix"][i]) // 16, int(rec["iy"][i]) // 16,
-                                           _morton(int(rec["ix"][i]) % 16, int(rec["iy"][i]) % 16),
+    want = sorted(range(n), key=lambda i: (int(rec["ix"][i]) // tx, int(rec["iy"][i]) // 16,
+                                           _morton(int(rec["ix"][i]) % tx, int(rec["iy"][i]) % 16),
                                            i))
     assert np.array_equal(got["k0"], np.array(want))
     assert np.array_equal(got, rec[want])
