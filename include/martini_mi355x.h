@@ -134,7 +134,9 @@ const char* mi_strerror(int code);
 
 /*
  * Number of (particle, cell) pairs the binning of these particles produces:
- * a cell is a 16 x 16 pixel tile times a 64-channel block.  Host-side count
+ * a cell is an 8 x 16 pixel tile times a 128-channel block when the cube has
+ * 128 channels or more (nc >= 128), else a 16 x 16 tile times a 64-channel
+ * block.  Host-side count
  * from host records (no device access); the value sizes the workspace.
  */
 int64_t mi_count_pairs_host(const mi_particle* h_particles, int64_t n,
