@@ -145,9 +145,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         for (int nb = 0; nb < 4; ++nb) acc[r][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     // Every wave holds the records of the next chunk and the sort indices of the
-    // one after.  A chunk is staged by the wave that had the fewest tile rows to
-    // multiply in the chunk before (it reaches the staging first); every wave
-    // picks it from the same row mask.
+    // one after.  A chunk is staged by the wave that had the fewest (row, block)
+    // products in the chunk before (it reaches the staging first); every wave
+    // picks it from the same row and channel-block masks.
     const int tkind = stage_table(P, order, it.begin, wtab, Tk);
     RecRegs next{};
     uint32_t next_idx = 0;
