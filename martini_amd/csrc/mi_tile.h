@@ -404,10 +404,8 @@ __device__ inline void fill_weights(const ChunkState& cs, int x0, int y0,
 
 // Evaluates S for the chunk's KC particles x CB channels of the cell and
 // hands the values to store2(k, channel1, s1, channel2, s2), two channels of
-// one particle at a time; particles k >= nk give 0, and so do channels outside
-// a particle's window, except for the Gaussian line with narrow channels (s <=
-// 1/2), whose series gives its own negligible tail there (channels beyond the
-// cube are never written out).  Thread t takes particle t % KC and the channels t / KC + 8 i, so its
+// one particle at a time; particles k >= nk and channels beyond the cube give
+// 0.  Thread t takes particle t % KC and the channels t / KC + 8 i, so its
 // record fields are read once.  Only the 16-channel blocks set in `blocks`
 // (wave-uniform) are evaluated; the others are left untouched.  The values
 // are line_flux's (for narrow spans the same series with its polynomial
@@ -460,8 +458,7 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     // Narrow spans: span_series with its D-dependent factors folded once per
     // particle (D = s), 0.5 amp (erf(m + D/2) - erf(m - D/2)) =
     // exp(-m^2) (e0 + e1 m^2 + e2 m^4 + e3 m^6); the same polynomial expanded.
-    // A particle beyond the chunk (not live) gets K = 0: all its values are 0.
-    const float D2 = s * s, K = live ? amp * 0.564189583547756287f * s : 0.0f;  // amp D / sqrt(pi)
+    const float D2 = s * s, K = amp * 0.564189583547756287f * s;  // amp D / sqrt(pi)
     const float e0 = K * (1.0f + D2 * (-1.0f / 12.0f + D2 * (1.0f / 160.0f - D2 * (1.0f / 2688.0f))));
     const float e1 = K * D2 * (1.0f / 6.0f + D2 * (-1.0f / 40.0f + D2 * (1.0f / 448.0f)));
     const float e2 = K * D2 * D2 * (1.0f / 120.0f - D2 * (1.0f / 672.0f));
@@ -471,35 +468,33 @@ __device__ inline void fill_spectra(const ChunkState& cs, int nk, int c0, int nc
     // the small integer 8 j from A is exact in the in-window range.
     const int kc = k0 - c0 - cl0;
     const float A = (float)kc + (fu - 0.5f);
+    // exp(-m^2) as exp2(-n^2) with n = m sqrt(log2 e): n = sn (A - 8 j), and the
+    // polynomial in m^2 = n^2 / log2 e with its coefficients scaled once.
+    constexpr float LOG2E = 1.44269504088896341f;
+    const float sn = s * 1.20112240878645110f;  // sqrt(log2 e)
+    const float g1 = e1 * (1.0f / LOG2E), g2 = e2 * (1.0f / (LOG2E * LOG2E)),
+                g3 = e3 * (1.0f / (LOG2E * LOG2E * LOG2E));
 #pragma unroll
     for (int b = 0; b < CB / 16; ++b) {
         if (!((blocks >> b) & 1u)) continue;  // uniform
         float v[PER_BLOCK];
 #pragma unroll
         for (int i = 0; i < PER_BLOCK; ++i) {
-            const float m = s * (A - (float)((NT / KC) * (b * PER_BLOCK + i))), m2 = m * m;
-            v[i] = __expf(-m2) * fmaf(fmaf(fmaf(e3, m2, e2), m2, e1), m2, e0);
+            const float n = sn * (A - (float)((NT / KC) * (b * PER_BLOCK + i))), n2 = n * n;
+            v[i] = __builtin_amdgcn_exp2f(-n2) * fmaf(fmaf(fmaf(g3, n2, g2), n2, g1), n2, e0);
         }
         if (s > 0.5f) {
             // wide channels: a line this narrow spans a few channels, and the
-            // erfc differences are taken only inside its window (0 outside)
+            // erfc differences are taken only inside its window
 #pragma unroll
             for (int i = 0; i < PER_BLOCK; ++i) {
                 const int o = (NT / KC) * (b * PER_BLOCK + i);
-                if (inside(o)) {
-                    const float m = s * (((float)(kc - o) + fu) - 0.5f);  // kc - o = k0 - channel
-                    v[i] = 0.5f * amp * erf_diff(m - 0.5f * s, m + 0.5f * s);
-                } else {
-                    v[i] = 0.0f;
-                }
+                if (!inside(o)) continue;
+                const float m = s * (((float)(kc - o) + fu) - 0.5f);  // kc - o = k0 - channel
+                v[i] = 0.5f * amp * erf_diff(m - 0.5f * s, m + 0.5f * s);
             }
         }
-        // Narrow channels: no window test.  Outside the window, in a block the
-        // window reaches, the series gives the line's own tail (below 1.1e-7
-        // of its peak channel, 0 once exp underflows), as the reference's
-        // every-channel spectrum has it.
-        const int o0 = (NT / KC) * (b * PER_BLOCK);
-        store2(k, cl0 + o0, v[0], cl0 + o0 + NT / KC, v[1]);
+        put(b, v[0], v[1]);
     }
 }
 
