@@ -23,6 +23,9 @@ FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # loop heads on 16-byte boundaries: the MFMA gather's chunk loop then takes
+    # fewer instruction fetches (about 0.5% of its time on a 1e7-particle cube)
+    "-falign-loops=16",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["mi_bin.hip", "mi_gather.hip", "mi_mfma.hip", "mi_prepare.hip", "mi_beam.hip",

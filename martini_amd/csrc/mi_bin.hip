@@ -90,6 +90,7 @@ hipError_t layout_for(const Grid& g, int64_t n, int64_t n_pairs, Layout* L) {
     L->item_idx = take(4 * (size_t)L->items_max);
     L->item_order = take(4 * (size_t)L->items_max);
     L->partials = take(sizeof(float) * (size_t)L->multi_max * CELL_VOXELS);
+    L->part_masks = take(sizeof(uint64_t) * (size_t)L->multi_max);
     L->status = take(4 * 8);
     // rocPRIM temporary storage: the larger of its two uses (the pair and the item sort).
     size_t t1 = 0, t2 = 0;

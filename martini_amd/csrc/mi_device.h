@@ -37,6 +37,36 @@ constexpr int CELL_VOXELS = 16384;  // one cell's (partial) tile, either shape
 static_assert(GeoSquare::PX * GeoSquare::CB == CELL_VOXELS &&
                   GeoWide::PX * GeoWide::CB == CELL_VOXELS,
               "both cell shapes hold the same number of voxels");
+// A cell's sub-tiles: one tile row (16 pixels) x SUB_CH channels each, 64 in
+// either shape.  A partial tile stores them one after another, each 1 KiB
+// ([row][channel block][pixel][SUB_CH]).  Bit sub_bit(row, block) of its mask
+// is set where the item wrote that sub-tile; the others hold nothing of this
+// call and are read as zero.  The bits follow the MFMA gather's waves: wave
+// w = row / 4 + (TX / 4) * (block / 4) owns bits 16 w .. 16 w + 15 (4 (row % 4)
+// + block % 4), so each wave stores its own 16 bits.
+constexpr int SUB_CH = 16;
+static_assert(GeoSquare::TX * (GeoSquare::CB / SUB_CH) == 64 &&
+                  GeoWide::TX * (GeoWide::CB / SUB_CH) == 64,
+              "one 64-bit sub-tile mask per partial tile");
+__host__ __device__ constexpr int sub_offset(int cb_blocks, int row, int block, int pixel) {
+    return ((row * cb_blocks + block) * 16 + pixel) * 16;
+}
+__host__ __device__ constexpr int sub_bit(int tx_shift, int row, int block) {
+    return 16 * ((row >> 2) + ((1 << tx_shift) >> 2) * (block >> 2)) + (row & 3) * 4 + (block & 3);
+}
+constexpr bool sub_bits_distinct(int tx_shift, int rows, int blocks) {
+    uint64_t seen = 0u;
+    for (int r = 0; r < rows; ++r)
+        for (int b = 0; b < blocks; ++b) {
+            const int bit = sub_bit(tx_shift, r, b);
+            if (bit < 0 || bit > 63 || ((seen >> bit) & 1u)) return false;
+            seen |= 1ull << bit;
+        }
+    return true;
+}
+static_assert(sub_bits_distinct(3, GeoWide::TX, GeoWide::CB / SUB_CH) &&
+                  sub_bits_distinct(4, GeoSquare::TX, GeoSquare::CB / SUB_CH),
+              "one mask bit per sub-tile");
 constexpr int TILE_MAX = 16;  // the longer tile side of either shape
 constexpr int TILE_Y = 16;    // tile columns, the same in both shapes
 static_assert(GeoSquare::TY == TILE_Y && GeoWide::TY == TILE_Y, "one row length");

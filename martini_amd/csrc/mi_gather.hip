@@ -34,12 +34,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
     const Item* __restrict__ items, const uint32_t* __restrict__ item_order,
     const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
-    float* __restrict__ cube, float* __restrict__ partials) {
+    float* __restrict__ cube, float* __restrict__ partials, uint64_t* __restrict__ part_masks) {
     if (status[ST_OVERFLOW]) return;
     if (blockIdx.x >= status[ST_ITEMS]) return;
     const Item it = items[item_order[blockIdx.x]];  // longest items first
     const CellPos cp = cell_pos(it.cell, g);
     const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
+    // This form writes its whole partial tile.
+    if (it.slot >= 0 && threadIdx.x == 0) part_masks[it.slot] = ~0ull;
 
     __shared__ __attribute__((aligned(16))) float Wt[KC][G::PX];
     // rows padded to CB + 4 floats: the S fill writes a column (32
@@ -182,24 +184,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
             }
         }
     } else {
-        float4* dst = reinterpret_cast<float4*>(partials + (size_t)it.slot * CELL_VOXELS);
+        static_assert(SUB_CH == 16, "four float4 per sub-tile row");
+        float* dst = partials + (size_t)it.slot * CELL_VOXELS + (cg & 3) * 4;
 #pragma unroll
         for (int jj = 0; jj < TILE_Y; ++jj)
-            dst[(pg * TILE_Y + jj) * CG + cg] =
+            *reinterpret_cast<float4*>(dst + sub_offset(G::CB / SUB_CH, pg, cg >> 2, jj)) =
                 make_float4(acc[jj][0], acc[jj][1], acc[jj][2], acc[jj][3]);
     }
 }
 
 // Sums the partial tiles of each split cell in item order and adds them to
-// the cube.  Block (cell, part): 256 threads x 4 voxels cover a 1024-voxel
-// part of the CELL_VOXELS = 16384-voxel tile; the item loop is unrolled so its loads are
-// independent.
+// the cube.  Block (cell, part): 256 threads x 4 voxels cover four 1 KiB
+// sub-tiles of the CELL_VOXELS = 16384-voxel tile, one per wave; the item loop
+// is unrolled so its loads are independent.  A partial's sub-tile that its item
+// did not write (its mask bit clear) counts as zero and is not read; voxels no
+// item wrote keep the cube's value.
 constexpr int RED_PARTS = CELL_VOXELS / (NT * 4);
 
 __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item_off,
                                                const uint32_t* __restrict__ multi_off,
                                                const uint32_t* __restrict__ status, Grid g,
                                                const float* __restrict__ partials,
+                                               const uint64_t* __restrict__ part_masks,
                                                float* __restrict__ cube) {
     if (status[ST_OVERFLOW]) return;
     const int cell = blockIdx.x / RED_PARTS;
@@ -208,24 +214,49 @@ __global__ __launch_bounds__(NT) void k_reduce(const uint32_t* __restrict__ item
     if (ni <= 1) return;
     const size_t tile = (size_t)CELL_VOXELS / 4;  // float4 per partial tile
     const float4* base = reinterpret_cast<const float4*>(partials) + (size_t)multi_off[cell] * tile;
+    const uint64_t* mk = part_masks + multi_off[cell];
     const int v4 = part * NT + threadIdx.x;  // float4 index within the tile
+    // v4 = (sub-tile * TILE_Y + pixel) * 4 + float4 within the 16 channels
+    static_assert(TILE_Y == 16 && SUB_CH == 16 && NT % 64 == 0, "one sub-tile per wave");
+    const int sub = v4 >> 6;  // row * blocks + block, the same in a wave
+    const int nblk = (1 << g.cb_shift) / SUB_CH;
+    const int prow = sub / nblk, jl = (v4 >> 2) & 15;
+    const int cl = (sub - prow * nblk) * SUB_CH + (v4 & 3) * 4;
+    const int bit = sub_bit(g.tx_shift, prow, sub - prow * nblk);
+    // An unwritten sub-tile's load is sent to the first 1 KiB of the partials
+    // (the same for every wave, so cached) and its bits cleared: no branch, and
+    // the four loads of a step stay independent.
+    const uint4* src = reinterpret_cast<const uint4*>(base);
+    const uint4* hot = reinterpret_cast<const uint4*>(partials) + (v4 & 63);
+    auto take = [&](uint32_t q, uint64_t m) {
+        const uint4 r = m ? src[q * tile + v4] : *hot;
+        const uint32_t keep = 0u - (uint32_t)m;
+        return make_float4(__uint_as_float(r.x & keep), __uint_as_float(r.y & keep),
+                           __uint_as_float(r.z & keep), __uint_as_float(r.w & keep));
+    };
     float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint64_t seen = 0u;
     uint32_t q = 0;
     for (; q + 4 <= ni; q += 4) {
-        const float4 a = base[(q + 0) * tile + v4], b = base[(q + 1) * tile + v4];
-        const float4 c = base[(q + 2) * tile + v4], e = base[(q + 3) * tile + v4];
+        const uint64_t m0 = mk[q + 0] >> bit & 1u, m1 = mk[q + 1] >> bit & 1u;
+        const uint64_t m2 = mk[q + 2] >> bit & 1u, m3 = mk[q + 3] >> bit & 1u;
+        const float4 a = take(q + 0, m0), b = take(q + 1, m1);
+        const float4 c = take(q + 2, m2), e = take(q + 3, m3);
         s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
         s0.x += b.x; s0.y += b.y; s0.z += b.z; s0.w += b.w;
         s0.x += c.x; s0.y += c.y; s0.z += c.z; s0.w += c.w;
         s0.x += e.x; s0.y += e.y; s0.z += e.z; s0.w += e.w;
+        seen |= m0 | m1 | m2 | m3;
     }
     for (; q < ni; ++q) {
-        const float4 a = base[q * tile + v4];
+        const uint64_t m0 = mk[q] >> bit & 1u;
+        const float4 a = take(q, m0);
         s0.x += a.x; s0.y += a.y; s0.z += a.z; s0.w += a.w;
+        seen |= m0;
     }
+    if (!seen) return;
     const CellPos cp = cell_pos(cell, g);
-    const int px = (v4 * 4) >> g.cb_shift, cl = (v4 * 4) & ((1 << g.cb_shift) - 1);
-    const int i = cp.x0 + px / TILE_Y, j = cp.y0 + px % TILE_Y;
+    const int i = cp.x0 + prow, j = cp.y0 + jl;
     if (i >= g.nx || j >= g.ny) return;
     float* dst = cube + ((size_t)i * g.ny + j) * g.nc + cp.c0 + cl;
     const float sv[4] = {s0.x, s0.y, s0.z, s0.w};
@@ -293,7 +324,8 @@ hipError_t gather_plain(const mi_particle* d_p, const Grid& g, const mi_cube_des
         hipLaunchKernelGGL(kernel, dim3((unsigned)L.items_max), dim3(NT), 0, s, d_p,
                            (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
                            (const uint32_t*)(ws + L.item_order), (const uint32_t*)(ws + L.status),
-                           wtab, g, d, d_cube, (float*)(ws + L.partials));
+                           wtab, g, d, d_cube, (float*)(ws + L.partials),
+                           (uint64_t*)(ws + L.part_masks));
     };
     if (g.wide)
         launch(k_gather_plain<GeoWide>);
@@ -307,7 +339,7 @@ hipError_t reduce_partials(const Grid& g, char* ws, const Layout& L, float* d_cu
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)(g.ncells * RED_PARTS)), dim3(NT), 0, s,
                        (const uint32_t*)(ws + L.item_off), (const uint32_t*)(ws + L.multi_off),
                        (const uint32_t*)(ws + L.status), g, (const float*)(ws + L.partials),
-                       d_cube);
+                       (const uint64_t*)(ws + L.part_masks), d_cube);
     return hipGetLastError();
 }
 

@@ -51,6 +51,7 @@ struct Layout {
     size_t item_keys, item_keys2;      // uint32[items_max]: sort keys of the launch order
     size_t item_idx, item_order;       // uint32[items_max]: item index; launch order
     size_t partials;                   // float[multi_max * CELL_VOXELS]
+    size_t part_masks;                 // uint64[multi_max]: the sub-tiles each partial holds
     size_t status;                     // uint32[8]
     size_t temp;                       // rocPRIM temporary storage
     size_t temp_bytes;

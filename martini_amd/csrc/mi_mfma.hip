@@ -106,7 +106,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const mi_particle* __restrict__ P, const uint32_t* __restrict__ order,
     const Item* __restrict__ items, const uint32_t* __restrict__ item_order,
     const uint32_t* __restrict__ status, const float4* __restrict__ wtab, Grid g, mi_cube_desc d,
-    float* __restrict__ cube, float* __restrict__ partials) {
+    float* __restrict__ cube, float* __restrict__ partials, uint64_t* __restrict__ part_masks) {
     if (status[ST_OVERFLOW]) return;
     if (blockIdx.x >= status[ST_ITEMS]) return;
     const Item it = items[item_order[blockIdx.x]];  // longest items first
@@ -123,6 +123,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // Waves: RG groups of four tile rows x CGN groups of four 16-channel blocks.
     constexpr int RG = G::TX / 4, CGN = G::CB / 64;
     static_assert(RG * CGN == NT / 64, "one (row group, channel group) per wave");
+    static_assert(SUB_CH == 16, "a sub-tile is one 16 x 16 product block");
     __shared__ uint32_t done_rows[RG];  // waves done reading each row group this chunk
 
     const int t = threadIdx.x;
@@ -239,13 +240,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         }
     }
 
-    // D layout of a 16x16 block: lane holds rows 4*kg..4*kg+3 (pixels j) of column m (channel).
+    // D layout of a 16x16 block: lane holds rows 4*kg..4*kg+3 (pixels j) of column m
+    // (channel).  Blocks that are all zero (no product reached them) are not
+    // written; a partial tile's mask records the blocks that were.
+    uint32_t mine = 0u;  // bit 4 r + nb: block (r, nb) written
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int prow = rh * 4 + r;  // tile row
         const int i = x0 + prow;
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb) {
+            const floatx4 v4 = acc[r][nb];
+            if (__ballot(v4[0] != 0.f || v4[1] != 0.f || v4[2] != 0.f || v4[3] != 0.f) == 0u)
+                continue;
+            mine |= 1u << (4 * r + nb);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int jl = kg * 4 + q;
@@ -255,11 +263,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                     const int j = y0 + jl, c = c0 + cl;
                     if (i < g.nx && j < g.ny && c < g.nc) cube[((size_t)i * g.ny + j) * g.nc + c] += v;
                 } else {
-                    partials[(size_t)it.slot * CELL_VOXELS + (prow * TILE_Y + jl) * G::CB + cl] = v;
+                    partials[(size_t)it.slot * CELL_VOXELS +
+                             sub_offset(G::CB / SUB_CH, prow, ch * 4 + nb, jl) + m] = v;
                 }
             }
         }
     }
+    static_assert(NT / 64 == 4, "four 16-bit parts per mask");
+    if (it.slot >= 0 && lane == 0)  // this wave's part: bits 16 * wave + 4 r + nb
+        reinterpret_cast<uint16_t*>(part_masks + it.slot)[wave] = (uint16_t)mine;
 }
 
 hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc& d, char* ws,
@@ -272,7 +284,8 @@ hipError_t gather_mfma(const mi_particle* d_p, const Grid& g, const mi_cube_desc
         hipLaunchKernelGGL(kernel, dim3((unsigned)L.items_max), dim3(NT), 0, s, d_p,
                            (const uint32_t*)(ws + L.vals2), (const Item*)(ws + L.items),
                            (const uint32_t*)(ws + L.item_order), (const uint32_t*)(ws + L.status),
-                           wtab, g, d, d_cube, (float*)(ws + L.partials));
+                           wtab, g, d, d_cube, (float*)(ws + L.partials),
+                           (uint64_t*)(ws + L.part_masks));
     };
     if (g.wide)
         launch(k_gather_mfma<GeoWide>);
