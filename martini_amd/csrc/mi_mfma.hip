@@ -29,6 +29,10 @@
 // are tabulated before the W fill.  Each wave clears the W rows it used right
 // after reading them, so the next fill finds them zero; every fill rewrites
 // the S blocks that the products will read.
+// At the end of an item, accumulator blocks that are all zero are not
+// written: a whole cell skips their read-modify-write of the cube, and a split
+// cell's partial tile receives only the 1 KiB sub-tiles written, with a mask
+// (16 bits per wave) that tells the reduction which to read.
 // Accumulation order is fixed: deterministic, no atomics.
 #include <hip/hip_runtime.h>
 
