@@ -40,8 +40,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const Item it = items[item_order[blockIdx.x]];  // longest items first
     const CellPos cp = cell_pos(it.cell, g);
     const int x0 = cp.x0, y0 = cp.y0, c0 = cp.c0;
-    // This form writes its whole partial tile.
-    if (it.slot >= 0 && threadIdx.x == 0) part_masks[it.slot] = ~0ull;
 
     __shared__ __attribute__((aligned(16))) float Wt[KC][G::PX];
     // rows padded to CB + 4 floats: the S fill writes a column (32
@@ -168,9 +166,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 make_float4(0.f, 0.f, 0.f, 0.f);
     }
 
+    // A thread's 16 pixels x 4 channels that are all zero (no particle reached
+    // them) are not written.  The four threads of a sub-tile (one tile row x 16
+    // channels) are neighbouring lanes; a partial tile's mask gets the sub-tiles
+    // with a nonzero value, and each wave stores the whole bytes of the mask
+    // that cover its rows (no two waves share a byte).
+    bool nz = false;
+#pragma unroll
+    for (int jj = 0; jj < TILE_Y; ++jj)
+        nz = nz || acc[jj][0] != 0.f || acc[jj][1] != 0.f || acc[jj][2] != 0.f ||
+             acc[jj][3] != 0.f;
+    const uint64_t any = __ballot(nz);
+    const bool quad = ((any >> (lane & ~3)) & 0xFu) != 0u;  // this sub-tile has a value
     const int i = x0 + pg;
     if (it.slot < 0) {
-        if (i < g.nx) {
+        if (i < g.nx && nz) {
 #pragma unroll
             for (int jj = 0; jj < TILE_Y; ++jj) {
                 const int j = y0 + jj;
@@ -185,11 +195,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
         }
     } else {
         static_assert(SUB_CH == 16, "four float4 per sub-tile row");
-        float* dst = partials + (size_t)it.slot * CELL_VOXELS + (cg & 3) * 4;
+        if (quad) {
+            float* dst = partials + (size_t)it.slot * CELL_VOXELS + (cg & 3) * 4;
 #pragma unroll
-        for (int jj = 0; jj < TILE_Y; ++jj)
-            *reinterpret_cast<float4*>(dst + sub_offset(G::CB / SUB_CH, pg, cg >> 2, jj)) =
-                make_float4(acc[jj][0], acc[jj][1], acc[jj][2], acc[jj][3]);
+            for (int jj = 0; jj < TILE_Y; ++jj)
+                *reinterpret_cast<float4*>(dst + sub_offset(G::CB / SUB_CH, pg, cg >> 2, jj)) =
+                    make_float4(acc[jj][0], acc[jj][1], acc[jj][2], acc[jj][3]);
+        }
+        // The wave's 16 sub-tiles: lanes 4 q .. 4 q + 3 hold row
+        // wave * WROWS + 4 q / CG, block (4 q % CG) / 4.
+        constexpr int TXS = __builtin_ctz(G::TX);
+        static_assert((1 << TXS) == G::TX && CG % 4 == 0, "tile rows a power of two");
+        uint64_t mine = 0u, owned = 0u;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint64_t bit = 1ull << sub_bit(TXS, wave * WROWS + 4 * q / CG, (4 * q % CG) / 4);
+            owned |= bit;
+            if ((any >> (4 * q)) & 0xFu) mine |= bit;
+        }
+        if (lane == 0) {
+            unsigned char* mb = reinterpret_cast<unsigned char*>(part_masks + it.slot);
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+                if ((owned >> (8 * b)) & 0xFFu) mb[b] = (unsigned char)(mine >> (8 * b));
+        }
     }
 }
 
