@@ -80,27 +80,32 @@ __device__ inline void put_split2(char* hi, char* lo, int k, int row1, float x1,
     *reinterpret_cast<__bf16*>(lo + o2) = l.y;
 }
 
-// put_split2 for rows given as 16 * block + r: the row's byte offset is the
-// block's 1 KiB step plus op_offset(r), so the block becomes a store's
-// immediate offset and the per-r offsets are shared by all blocks.
-__device__ inline void put_split2_blocks(char* hi, char* lo, int k, int row1, float x1, int row2,
-                                         float x2) {
-    const floatx2 x = {x1, x2};
-    const bf16x2 h = __builtin_convertvector(x, bf16x2);
-    const floatx2 hf = __builtin_convertvector(h, floatx2);
-    const bf16x2 l = __builtin_convertvector(floatx2{x1 - hf.x, x2 - hf.y}, bf16x2);
-    const int kb = k_bits(k);
-    const int o1 = (row1 >> 4) * (16 * KROW) + op_offset(row1 & 15, kb);
-    const int o2 = (row2 >> 4) * (16 * KROW) + op_offset(row2 & 15, kb);
-    *reinterpret_cast<__bf16*>(hi + o1) = h.x;
-    *reinterpret_cast<__bf16*>(lo + o1) = l.x;
-    *reinterpret_cast<__bf16*>(hi + o2) = h.y;
-    *reinterpret_cast<__bf16*>(lo + o2) = l.y;
-}
-
 // The 8 BF16 values of k-group kg of `row`.
 __device__ inline bf16x8 get8(const char* tile, int row, int kg) {
     return *reinterpret_cast<const bf16x8*>(tile + row * KROW + (((kg ^ (row >> 2)) & 3) << 4));
+}
+
+// put_split2 for the S fill, whose lanes l and l ^ 1 hold the same two rows of
+// particles k and k ^ 1 (rows given as 16 * block + r: the block's 1 KiB step
+// plus op_offset(r)).  After one lane swap (DPP quad_perm [1, 0, 3, 2]) the
+// even lane writes row1 and the odd lane row2, each as the pair (k even, k odd)
+// -- adjacent BF16 slots of one 4-byte word -- in both tiles: one paired
+// 4-byte store per lane (the hi and lo tiles lie a fixed distance apart)
+// instead of four 2-byte ones.  Every lane of the wave must call it.
+__device__ inline void put_split2_pairs(char* hi, char* lo, int k, int row1, float x1, int row2,
+                                        float x2) {
+    constexpr int SWAP = 0xB1;  // quad_perm [1, 0, 3, 2]: lanes l and l ^ 1
+    const float y1 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x1), SWAP, 0xF, 0xF, false));
+    const float y2 = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x2), SWAP, 0xF, 0xF, false));
+    const bool even = (k & 1) == 0;
+    const int row = even ? row1 : row2;
+    const floatx2 x = even ? floatx2{x1, y1} : floatx2{y2, x2};  // particles k & ~1, k | 1
+    const bf16x2 h = __builtin_convertvector(x, bf16x2);
+    const floatx2 hf = __builtin_convertvector(h, floatx2);
+    const bf16x2 l = __builtin_convertvector(floatx2{x.x - hf.x, x.y - hf.y}, bf16x2);
+    const int o = (row >> 4) * (16 * KROW) + op_offset(row & 15, k_bits(k & ~1));
+    *reinterpret_cast<bf16x2*>(hi + o) = h;
+    *reinterpret_cast<bf16x2*>(lo + o) = l;
 }
 
 // Three waves per SIMD: the 64 accumulators then live in VGPRs and three work
@@ -191,7 +196,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void k_
                      });
         fill_spectra<G::CB>(cs, nk, c0, g.nc, d,
                      [&](int k, int c1, float s1, int c2, float s2) {
-                         put_split2_blocks(Bh, Bl, k, c1, s1, c2, s2);
+                         put_split2_pairs(Bh, Bl, k, c1, s1, c2, s2);
                      },
                      chanmask);
         __syncthreads();
